@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Throughput of the beamforming hot path on one MI355X.
+
+Workload (one step = one pass of the hot path over one batch): Synthetic-point
+DAS with natural-cubic interpolation, 75 plane waves over +-16 deg encoded as
+synthetic points at -10 m, 128 receive elements (0.3 mm pitch, f/1.5
+acceptance), 4096 float32 samples per channel at 40 MHz, a 512 x 512 pixel grid
+over z 2-50 mm and x +-19 mm, a batch of 16 independently seeded frames (50
+point scatterers + 1e4 speckle scatterers each, 30 dB SNR).  RF and geometry
+are resident in HBM before the timed region; each step runs the spline
+prefilter, the delay tables and the gather kernel and writes the 16 images.
+
+Prints one JSON line (metric / value in Gsamples/s of tx*rx*px*frames).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gsamples/s (tx·rx·px) cubic DAS + SLSC px/s on 1 MI355X vs plain GPU & CPU"
+HBM_PEAK_GBPS = 8000.0
+
+NTX, NRX, NT, NX, NZ = 75, 128, 4096, 512, 512
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_geometry():
+    from pycbf_amd import sim
+    xele = sim.linear_array(NRX)
+    steers = np.radians(np.linspace(-16, 16, NTX))
+    _, _, pnts = sim.grid(NX, NZ, (-19e-3, 19e-3), (2e-3, 50e-3))
+    params = sim.pw_synthetic(steers, xele, pnts)
+    return xele, steers, pnts, params
+
+
+def make_rf(steers, xele, nframes, device, seed=1234):
+    import torch
+    from pycbf_amd import sim
+    rf = torch.empty((nframes, NTX, NRX, NT), dtype=torch.float32, device=device)
+    for f in range(nframes):
+        xs, zs, amp = sim.scatterers(50, 10_000, (-19e-3, 19e-3), (2e-3, 50e-3), seed + f)
+        rf[f] = sim.simulate_rf_torch(steers, xele, NT, xs, zs, amp, device, seed=seed + f)
+        log(f"simulated frame {f + 1}/{nframes}")
+    torch.cuda.synchronize()
+    return rf
+
+
+def cpu_baseline(rf0, params, stride, threads):
+    """Reference C cubic DAS (oracle/_ref, pycbf/cpu/pycbf.c) -- or the oracle
+    restatement when the reference build is absent -- on frame 0 over a strided
+    pixel subset, channels split over `threads` host threads with private
+    accumulators.  Returns (Gsamples/s, threads, kind, sample description)."""
+    import ctypes as ct
+
+    import oracle
+    pnts = params["pnts"][::stride].copy()
+    p = params
+    tabs = oracle.synthetic_tables(p["ovectx"], p["nvectx"], p["t0tx"], p["alatx"], p["doftx"],
+                                   p["ovecrx"], p["nvecrx"], p["alarx"], p["c0"], pnts)
+    tautx, apodtx, taurx, apodrx = tabs
+    npx = pnts.shape[0]
+    R = oracle.ref()
+    kind = "reference" if R is not None else "port"
+    fn = R.beamform_cubic if R is not None else oracle.lib().orc_bf_cubic
+    pf = ct.POINTER(ct.c_float)
+    rf0 = np.ascontiguousarray(rf0, np.float32)
+    outs = [np.zeros(npx, np.float32) for _ in range(threads)]
+    dt = float(np.float32(1.0 / 40e6))
+    chans = [(t, r) for t in range(NTX) for r in range(NRX)]
+
+    def work(k):
+        o = outs[k].ctypes.data_as(pf)
+        for idx in range(k, len(chans), threads):
+            t, r = chans[idx]
+            fn(0.0, dt, NT, rf0[t, r].ctypes.data_as(pf), npx, 1e-2,
+               tautx[t].ctypes.data_as(pf), apodtx[t].ctypes.data_as(pf),
+               taurx[r].ctypes.data_as(pf), apodrx[r].ctypes.data_as(pf), o)
+
+    ths = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    t0 = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    samples = NTX * NRX * npx
+    desc = (f"frame 0, {NTX}x{NRX} channels, every {stride}th of {NX * NZ} pixels "
+            f"({npx} px, {samples / 1e6:.0f} M samples), cubic, {wall:.2f} s wall")
+    return samples / wall / 1e9, threads, kind, desc
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=8)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the hot path shards over frames / pixels with no exchange and is not run "
+              "across GPUs here", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pycbf_amd.gpu import SyntheticBeamformer
+    from pycbf_amd.gpu import _engine as E
+
+    dev = torch.device("cuda", 0)
+    xele, steers, pnts, params = workload_geometry()
+    log(f"workload: {NTX} tx x {NRX} rx x {NX}x{NZ} px, nt={NT}, {args.frames} frames")
+    rf = make_rf(steers, xele, args.frames, dev)
+    bf = SyntheticBeamformer(**params, t0=0.0, dt=1.0 / 40e6, nt=NT, interp={"kind": "cubic"},
+                             sumtype="tx_and_rx", device=dev)
+    out = torch.empty((args.frames, NX * NZ), dtype=torch.float32, device=dev)
+    for i in range(args.warmup):
+        bf(rf, out_as_numpy=False, buffer=out)
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup}")
+    E.lib.pycbf_profile_reset()
+    E.lib.pycbf_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bf(rf, out_as_numpy=False, buffer=out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    E.lib.pycbf_profile_enable(0)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    samples = NTX * NRX * NX * NZ * args.frames
+    value = samples / (ms_step * 1e-3) / 1e9
+    log(f"{ms_step:.3f} ms/step, {value:.1f} Gsamples/s")
+
+    g_ms, g_n = E.profile_read("das_gather")
+    s_ms, s_n = E.profile_read("spline_coefs")
+    t_ms, t_n = E.profile_read("synthetic_tables")
+    gather_ms = g_ms / max(g_n, 1)
+    npx = NX * NZ
+    algo_bytes = (16.0 * samples + 4.0 * NTX * NRX * NT * args.frames + 4.0 * 2 * npx
+                  + 4.0 * npx * args.frames)
+    achieved = algo_bytes / (gather_ms * 1e-3) / 1e9
+    log(f"gather {gather_ms:.3f} ms/launch, spline {s_ms / max(s_n, 1):.3f} ms, "
+        f"tables {t_ms / max(t_n, 1):.3f} ms")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "das_gather_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("frames") == args.frames:
+            traffic = tr.get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rf0 = rf[0].cpu().numpy()
+        rate, used, kind, desc = cpu_baseline(rf0, params, args.cpu_stride, threads)
+        cpu = {"value": round(rate, 4), "unit": "Gsamples/s", "cores": used, "kind": kind,
+               "sample": desc}
+        log(f"cpu baseline: {rate:.4f} Gsamples/s on {used} threads ({kind})")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded plane-wave channel data: 50 points + 1e4 speckle per frame)",
+        "config": {"workload": f"Synthetic cubic DAS {NTX}tx x {NRX}rx x {NX}x{NZ}px, "
+                               f"nt={NT}, {args.frames} frames, sumtype tx_and_rx",
+                   "ntx": NTX, "nrx": NRX, "nt": NT, "grid": [NX, NZ], "frames": args.frames,
+                   "interp": "cubic"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic, "kernel": "das_gather",
+                     "kernel_ms": round(gather_ms, 4)},
+        "cpu_baseline": cpu,
+        "phases_ms": {"das_gather": round(gather_ms, 4),
+                      "spline_coefs": round(s_ms / max(s_n, 1), 4),
+                      "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

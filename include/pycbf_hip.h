@@ -1,0 +1,176 @@
+/*
+ * pycbf_hip.h -- C-ABI of libpycbf_hip.so, the MI355X (gfx950) beamforming
+ * engine that replaces pycbf's CuPy-JIT kernel module.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success, otherwise a nonzero status
+ *     (a hipError_t value, or PYCBF_EINVAL for argument errors).  The message
+ *     of the last failure on the calling thread is pycbf_last_error().
+ *   - Array arguments are caller-owned DEVICE pointers unless the comment says
+ *     "host".  Layouts are C-contiguous float32 as in pycbf.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Entry points enqueue
+ *     work and return without synchronising.
+ *   - Outputs are WRITTEN (not accumulated); no zero-filled output is needed.
+ *   - `workspace` is caller-owned device scratch of at least the size returned
+ *     by the matching *_workspace_bytes query; its contents are not preserved
+ *     between calls.  Calls that share a workspace must not overlap.
+ *
+ * Reference interfaces replaced (paths relative to the wewightman/pycbf tree):
+ *   pycbf/gpu/__engines__.py:9-26        CuPy RawModule + kernel handles
+ *   pycbf/gpu/__engines__.cu:360-374     struct xInfo / RFInfo (same layout)
+ *   pycbf/gpu/__engines__.cu:506-864     das_bmode_{synthetic,tabbed}_{korder_cubic,multi_interp}
+ *   pycbf/gpu/__advanced_engines__.cu:17-61  calc_dmas
+ *   pycbf/cpu/pycbf.h:23-54              tie_knots1D_fixed / beamform_cubic / beamform_nearest
+ */
+#ifndef PYCBF_HIP_H
+#define PYCBF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYCBF_ABI_VERSION 1
+#define PYCBF_EINVAL 1001
+
+/* interp kinds: 0..3 keep pycbf's interp_keys (pycbf/gpu/__core_bf_classes__.py:215,332) */
+#define PYCBF_INTERP_NEAREST 0
+#define PYCBF_INTERP_LINEAR 1
+#define PYCBF_INTERP_AKIMA 2
+#define PYCBF_INTERP_MAKIMA 3
+#define PYCBF_INTERP_KORDER_CUBIC 4
+#define PYCBF_INTERP_CUBIC 5
+
+/* sumtype flags, pycbf/gpu/__core_bf_classes__.py:173,296 */
+#define PYCBF_SUM_NONE 0
+#define PYCBF_SUM_TX_ONLY 1
+#define PYCBF_SUM_RX_ONLY 2
+#define PYCBF_SUM_TX_AND_RX 3
+
+/* DMAS kinds: 0 and 1 keep pycbf's dmasflag (pycbf/gpu/__advanced_bf_classes__.py:37-38) */
+#define PYCBF_DMAS_SSR 0
+#define PYCBF_DMAS_POWER 1
+#define PYCBF_DMAS_PSSR 2
+
+/* floats per transmit / receive element in the synthetic geometry packs */
+#define PYCBF_TXPACK_FLOATS 16
+#define PYCBF_RXPACK_FLOATS 8
+
+/* struct xInfo, pycbf/gpu/__engines__.cu:360-364 (12 bytes) */
+typedef struct pycbf_xinfo {
+    float x0;   /* first sample time t0 */
+    float dx;   /* sample period dt */
+    int32_t nx; /* samples per channel nt */
+} pycbf_xinfo;
+
+/* struct RFInfo, pycbf/gpu/__engines__.cu:369-374 (32 bytes, same offsets) */
+typedef struct pycbf_rfinfo {
+    int64_t ntx;       /* transmit events */
+    int64_t nrx;       /* receive channels */
+    int32_t ndim;      /* 2 or 3 for Synthetic, 0 for Tabbed */
+    pycbf_xinfo tinfo; /* time axis */
+} pycbf_rfinfo;
+
+/* ABI version (PYCBF_ABI_VERSION) and last error of the calling thread. */
+int pycbf_abi_version(void);
+const char *pycbf_last_error(void);
+
+/* Bytes of workspace the DAS entry points need for this problem.
+ * interp_param is usf for nearest, k for korder_cubic, ignored otherwise.
+ * synthetic != 0 sizes pycbf_das_synthetic (it adds the delay tables). */
+size_t pycbf_das_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
+                                 int interp_kind, int interp_param, int synthetic);
+
+/* Tabbed DAS.  Replaces das_bmode_tabbed_multi_interp (pycbf/gpu/__engines__.cu:813-864)
+ * and das_bmode_tabbed_korder_cubic (:726-778), with the CPU kernels' semantics for
+ * cubic / nearest(usf) (pycbf/cpu/pycbf.c:179-271): weight w = apodtx*apodrx, a
+ * sample contributes w*interp(rf[f,tx,rx,:], tautx+taurx) when |w| >= thresh.
+ *   rf       (nfr, ntx, nrx, nt)
+ *   tautx, apodtx (ntx, np); taurx, apodrx (nrx, np)
+ *   perm     optional (np) int32 pixel processing order (NULL = identity); it
+ *            only changes scheduling, never the result layout
+ *   S        (k, k) derivative operator for korder_cubic, else NULL
+ *   out      (nfr, ntx, nrx, np) | (nfr, nrx, np) | (nfr, ntx, np) | (nfr, np) by sumflag */
+int pycbf_das_tabbed(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                     const float *tautx, const float *apodtx, const float *taurx,
+                     const float *apodrx, const int32_t *perm, int64_t np, int interp_kind,
+                     int interp_param, const float *S, float thresh, int sumflag, float *out,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* HOST function.  Builds the per-element geometry packs consumed on the device
+ * from pycbf's Synthetic parameters (pycbf/__bf_base_classes__.py:141-183, all
+ * host arrays).  The acceptance tests "acos(v) > ala" of
+ * pycbf/gpu/__engines__.cu:412,427,468 become exact comparisons v < T with T
+ * derived from ala in double precision.
+ *   txpack: ntx * PYCBF_TXPACK_FLOATS floats, rxpack: nrx * PYCBF_RXPACK_FLOATS. */
+int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,
+                         const float *t0tx, const float *alatx, const float *doftx, int64_t nrx,
+                         const float *ovecrx, const float *nvecrx, const float *alarx,
+                         float *txpack, float *rxpack);
+
+/* Synthetic-point DAS.  Replaces das_bmode_synthetic_multi_interp
+ * (pycbf/gpu/__engines__.cu:620-694) and das_bmode_synthetic_korder_cubic
+ * (:506-581, whose k is passed as an int here).  Delays and apodisations come
+ * from calc_tautx_apodtx / calc_taurx_apodrx (:382-470) evaluated on the device.
+ *   txpack, rxpack  device copies of pycbf_synthetic_pack output
+ *   pvec            (np, ndim) pixel positions; other arguments as pycbf_das_tabbed */
+int pycbf_das_synthetic(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                        const float *txpack, const float *rxpack, float c0, const float *pvec,
+                        const int32_t *perm, int64_t np, int interp_kind, int interp_param,
+                        const float *S, float thresh, int sumflag, float *out, void *workspace,
+                        size_t workspace_bytes, void *stream);
+
+/* Delay / apodisation tables from the synthetic geometry (the per-sample
+ * calc_tautx_apodtx / calc_taurx_apodrx of pycbf/gpu/__engines__.cu:542-561). */
+int pycbf_synthetic_tables(const pycbf_rfinfo *info, const float *txpack, const float *rxpack,
+                           float c0, const float *pvec, int64_t np, float *tautx, float *apodtx,
+                           float *taurx, float *apodrx, void *stream);
+
+/* HOST function.  Spatially compact processing order for np pixel positions
+ * (host pvec (np, ndim)) written to host perm (np).  No reference counterpart:
+ * the reference launches one thread per sample in pixel-index order. */
+int pycbf_tile_order(const float *pvec, int64_t np, int ndim, int32_t *perm);
+
+/* Natural cubic spline of each channel (tie_knots1D_fixed, pycbf/cpu/pycbf.c:114-166)
+ * as per-interval cubic coefficients: coef (nch, nt, 4) so that on interval j with
+ * t = (x - t0 - j*dt)/dt the value is ((c3 t + c2) t + c1) t + c0.  The last
+ * interval (j = nt-1) zero-pads the neighbour the reference reads out of range. */
+int pycbf_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *coef,
+                       void *workspace, size_t workspace_bytes, void *stream);
+
+/* Natural spline second derivatives y2 (nch, nt), bit-identical to tie_knots1D_fixed. */
+int pycbf_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *y2,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* Workspace bytes for pycbf_spline_coefs / pycbf_spline_y2. */
+size_t pycbf_spline_workspace_bytes(int64_t nch, int32_t nt);
+
+/* Sample index each (tx, rx, px) selects, for parity tests of index selection:
+ * idx (ntx, nrx, np) int32, -1 where the sample falls outside the trace.
+ * kind nearest uses pycbf/cpu/pycbf.c:264 (with dt/usf), the others the knot
+ * index of pycbf/cpu/pycbf.c:81. */
+int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const float *taurx,
+                        int64_t np, int interp_kind, int usf, int32_t *idx, void *stream);
+
+/* DMAS over the separated axis.  Replaces calc_dmas (pycbf/gpu/__advanced_engines__.cu:17-61)
+ * but uses the lag values in `lags` (the reference used 0..nlag-1).
+ *   im  (nfr, nsep, np) delayed, partially summed images
+ *   out (nfr, np) if sumlags else (nfr, nlag, np)
+ *   dmastype PYCBF_DMAS_SSR: sum sign(q)sqrt|q|; POWER: sum q; PSSR: sign(S)sqrt|S| of the power sum */
+int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
+               int nlag, int dmastype, int sumlags, float *out, void *stream);
+
+/* Per-kernel timing: when enabled, instrumented launches record HIP events on
+ * their stream; pycbf_profile_read synchronises them and returns the summed
+ * duration (ms) and launch count of kernel `kernel` ("das_gather",
+ * "spline_coefs", "synthetic_tables", ...).  No reference counterpart. */
+void pycbf_profile_enable(int on);
+void pycbf_profile_reset(void);
+int pycbf_profile_read(const char *kernel, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYCBF_HIP_H */

@@ -1,0 +1,446 @@
+// extern "C" entry points of libpycbf_hip.so (declared in include/pycbf_hip.h):
+// argument validation, workspace planning, prefilter selection and launch.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+namespace pycbf {
+
+static thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    return fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ plans ---
+struct DasPlan {
+    int gk = GK_LINEAR;
+    int64_t nch = 0;       // frames * ntx * nrx channels
+    size_t coef = 0;       // float4 coefficient table
+    size_t uscr = 0;       // spline sweep scratch
+    size_t y2 = 0;         // spline second derivatives (nearest, usf > 1)
+    size_t up = 0;         // upsampled channels (nearest, usf > 1)
+    size_t tabs = 0;       // synthetic delay / apodisation tables
+    size_t part = 0;       // split partial sums
+    int nsplit = 1;
+    size_t total = 0;
+};
+
+static int frames_per_pass_host(int64_t nfr)
+{
+    if (nfr >= 16) return 16;
+    if (nfr >= 8) return 8;
+    if (nfr >= 4) return 4;
+    if (nfr >= 2) return 2;
+    return 1;
+}
+
+static int choose_split(int64_t np, int64_t nfr, int64_t na)
+{
+    const int64_t blocks = ((np + 255) / 256) * ((nfr + frames_per_pass_host(nfr) - 1) /
+                                                  frames_per_pass_host(nfr));
+    const int64_t target = 2048;  // 8 workgroups per CU on 256 CUs
+    int64_t s = (target + blocks - 1) / blocks;
+    if (s < 1) s = 1;
+    if (s > na) s = na;
+    if (s > 64) s = 64;
+    return (int)s;
+}
+
+static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind, int param,
+                        int synthetic, int sumflag)
+{
+    DasPlan p;
+    const int64_t nt = info->tinfo.nx;
+    p.nch = nfr * info->ntx * info->nrx;
+    switch (kind) {
+    case PYCBF_INTERP_NEAREST:
+        p.gk = GK_NEAREST;
+        if (param > 1) {
+            p.y2 = align_up((size_t)p.nch * nt * sizeof(float), 256);
+            p.uscr = spline_scratch_bytes(p.nch, (int32_t)nt);
+            p.up = align_up((size_t)p.nch * ((size_t)param * (nt - 1) + 1) * sizeof(float), 256);
+        }
+        break;
+    case PYCBF_INTERP_LINEAR:
+        p.gk = GK_LINEAR;
+        break;
+    case PYCBF_INTERP_CUBIC:
+        p.gk = GK_CUBIC;
+        p.coef = align_up((size_t)p.nch * nt * sizeof(float4), 256);
+        p.uscr = spline_scratch_bytes(p.nch, (int32_t)nt);
+        break;
+    case PYCBF_INTERP_KORDER_CUBIC:
+        p.gk = GK_KORDER;
+        p.coef = align_up((size_t)p.nch * nt * sizeof(float4), 256);
+        break;
+    default:  // akima, makima
+        p.gk = GK_AKIMA;
+        p.coef = align_up((size_t)p.nch * nt * sizeof(float4), 256);
+        break;
+    }
+    if (synthetic)
+        p.tabs = 4 * align_up((size_t)(info->ntx + info->nrx) * np * sizeof(float), 256);
+    // partial sums exist only for tx_and_rx (receivers outer); size them whenever
+    // sumflag < 0 (workspace query without a sumtype) or tx_and_rx
+    if (sumflag < 0 || sumflag == PYCBF_SUM_TX_AND_RX) {
+        p.nsplit = choose_split(np, nfr, info->nrx);
+        if (p.nsplit > 1) p.part = align_up((size_t)p.nsplit * nfr * np * sizeof(float), 256);
+    } else {
+        const int64_t na = (sumflag == PYCBF_SUM_RX_ONLY) ? info->ntx : info->nrx;
+        p.nsplit = choose_split(np, nfr, na);
+    }
+    p.total = p.coef + p.uscr + p.y2 + p.up + p.tabs + p.part + 1024;
+    return p;
+}
+
+static int validate_common(const pycbf_rfinfo *info, const float *rf, int64_t nfr, int64_t np,
+                           int kind, int param, const float *S, int sumflag, const float *out)
+{
+    if (!info) return fail(PYCBF_EINVAL, "info is NULL");
+    if (info->ntx < 1 || info->nrx < 1) return fail(PYCBF_EINVAL, "ntx and nrx must be >= 1");
+    if (info->tinfo.nx < 1) return fail(PYCBF_EINVAL, "nt must be >= 1");
+    if (!(info->tinfo.dx > 0.0f) || !isfinite(info->tinfo.dx) || !isfinite(info->tinfo.x0))
+        return fail(PYCBF_EINVAL, "dt must be finite and > 0 and t0 finite");
+    if (nfr < 1 || np < 1) return fail(PYCBF_EINVAL, "nfr and np must be >= 1");
+    if ((info->ntx * info->nrx) > 0x7fffffffLL)
+        return fail(PYCBF_EINVAL, "ntx*nrx must fit in int32");
+    if (!rf || !out) return fail(PYCBF_EINVAL, "rf and out must be device pointers");
+    if (sumflag < PYCBF_SUM_NONE || sumflag > PYCBF_SUM_TX_AND_RX)
+        return fail(PYCBF_EINVAL, "sumflag %d out of range", sumflag);
+    const int nt = info->tinfo.nx;
+    switch (kind) {
+    case PYCBF_INTERP_NEAREST:
+        if (param < 1) return fail(PYCBF_EINVAL, "usf must be >= 1");
+        if ((int64_t)param * (nt - 1) + 1 > 0x7fffffffLL)
+            return fail(PYCBF_EINVAL, "usf*(nt-1)+1 must fit in int32");
+        break;
+    case PYCBF_INTERP_LINEAR:
+    case PYCBF_INTERP_CUBIC:
+        break;
+    case PYCBF_INTERP_KORDER_CUBIC:
+        if (param < 4 || (param % 2) != 0)
+            return fail(PYCBF_EINVAL, "'k' for korder_cubic must be at least four and even");
+        if (param > nt) return fail(PYCBF_EINVAL, "k (%d) must not exceed nt (%d)", param, nt);
+        if (param > 128) return fail(PYCBF_EINVAL, "k (%d) must be <= 128", param);
+        if (!S) return fail(PYCBF_EINVAL, "korder_cubic needs the (k, k) S operator");
+        break;
+    case PYCBF_INTERP_AKIMA:
+    case PYCBF_INTERP_MAKIMA:
+        if (nt < 5) return fail(PYCBF_EINVAL, "akima / makima need nt >= 5");
+        break;
+    default:
+        return fail(PYCBF_EINVAL, "unknown interp kind %d", kind);
+    }
+    return 0;
+}
+
+// Builds the gather source for `kind` into the workspace and runs the gather.
+static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const float *ttx,
+                   const float *atx, const float *trx, const float *arx, const int32_t *perm,
+                   int64_t np, int kind, int param, const float *S, float thresh, int sumflag,
+                   float *out, const DasPlan &plan, Carver &cv, hipStream_t s)
+{
+    const int nt = info->tinfo.nx;
+    const float t0 = info->tinfo.x0, dt = info->tinfo.dx;
+    DasArgs a{};
+    a.ttx = ttx;
+    a.atx = atx;
+    a.trx = trx;
+    a.arx = arx;
+    a.perm = perm;
+    a.np = np;
+    a.ntx = (int)info->ntx;
+    a.nrx = (int)info->nrx;
+    a.nfr = (int)nfr;
+    a.t0 = t0;
+    a.dt = dt;
+    a.rdt = 1.0f / dt;
+    a.tmax = t0 + dt * (float)(nt - 1);
+    a.thresh = thresh;
+    a.sumflag = sumflag;
+    a.out = out;
+    a.src = rf;
+    a.ch_stride = nt;
+    a.nsrc = nt;
+    int rc = 0;
+    switch (kind) {
+    case PYCBF_INTERP_NEAREST:
+        if (param > 1) {
+            float *y2 = (float *)cv.take(plan.y2);
+            float *u = (float *)cv.take(plan.uscr);
+            float *up = (float *)cv.take(plan.up);
+            if (!y2 || !u || !up) return fail(PYCBF_EINVAL, "workspace too small");
+            if ((rc = launch_spline_y2(rf, plan.nch, nt, dt, u, y2, s))) return rc;
+            if ((rc = launch_upsample(rf, y2, plan.nch, nt, t0, dt, param, up, s))) return rc;
+            const int ntu = param * (nt - 1) + 1;
+            a.src = up;
+            a.ch_stride = ntu;
+            a.nsrc = ntu;
+            a.dt = dt / (float)param;
+            a.rdt = 1.0f / a.dt;
+        }
+        break;
+    case PYCBF_INTERP_LINEAR:
+        break;
+    case PYCBF_INTERP_CUBIC: {
+        float4 *coef = (float4 *)cv.take(plan.coef);
+        float *u = (float *)cv.take(plan.uscr);
+        if (!coef || !u) return fail(PYCBF_EINVAL, "workspace too small");
+        if ((rc = launch_spline_coefs(rf, plan.nch, nt, dt, u, coef, s))) return rc;
+        a.src = (const float *)coef;
+        a.ch_stride = 4 * (int64_t)nt;
+        break;
+    }
+    case PYCBF_INTERP_KORDER_CUBIC: {
+        float4 *coef = (float4 *)cv.take(plan.coef);
+        if (!coef) return fail(PYCBF_EINVAL, "workspace too small");
+        if ((rc = launch_korder_coefs(rf, plan.nch, nt, S, param, coef, s))) return rc;
+        a.src = (const float *)coef;
+        a.ch_stride = 4 * (int64_t)nt;
+        break;
+    }
+    default: {  // akima / makima
+        float4 *coef = (float4 *)cv.take(plan.coef);
+        if (!coef) return fail(PYCBF_EINVAL, "workspace too small");
+        if ((rc = launch_akima_coefs(rf, plan.nch, nt, kind == PYCBF_INTERP_MAKIMA, coef, s)))
+            return rc;
+        a.src = (const float *)coef;
+        a.ch_stride = 4 * (int64_t)nt;
+        break;
+    }
+    }
+    float *part = nullptr;
+    if (plan.part) {
+        part = (float *)cv.take(plan.part);
+        if (!part) return fail(PYCBF_EINVAL, "workspace too small");
+    }
+    return launch_das(plan.gk, a, plan.nsplit, part, s);
+}
+
+static float acos_threshold(float ala)
+{
+    // returns T with  acos(v) > ala  <=>  v < T  for every float v in [-1, 1]
+    if (ala != ala) return -INFINITY;
+    const double a = (double)ala;
+    if (a < 0.0) return INFINITY;
+    if (a > M_PI) return -INFINITY;
+    const double c = cos(a);
+    float T = (float)c;
+    if ((double)T < c) T = nextafterf(T, INFINITY);
+    return T;
+}
+
+}  // namespace pycbf
+
+using namespace pycbf;
+
+extern "C" {
+
+int pycbf_abi_version(void) { return PYCBF_ABI_VERSION; }
+
+const char *pycbf_last_error(void) { return g_err; }
+
+size_t pycbf_das_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
+                                 int interp_kind, int interp_param, int synthetic)
+{
+    if (!info || nfr < 1 || np < 1) return 0;
+    return plan_das(info, nfr, np, interp_kind, interp_param, synthetic, -1).total;
+}
+
+int pycbf_das_tabbed(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const float *tautx,
+                     const float *apodtx, const float *taurx, const float *apodrx,
+                     const int32_t *perm, int64_t np, int interp_kind, int interp_param,
+                     const float *S, float thresh, int sumflag, float *out, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    int rc = validate_common(info, rf, nfr, np, interp_kind, interp_param, S, sumflag, out);
+    if (rc) return rc;
+    if (!tautx || !apodtx || !taurx || !apodrx)
+        return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
+    const DasPlan plan = plan_das(info, nfr, np, interp_kind, interp_param, 0, sumflag);
+    if (workspace_bytes < plan.total || !workspace)
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
+                    workspace_bytes);
+    Carver cv(workspace, workspace_bytes);
+    return run_das(info, rf, nfr, tautx, apodtx, taurx, apodrx, perm, np, interp_kind,
+                   interp_param, S, thresh, sumflag, out, plan, cv, (hipStream_t)stream);
+}
+
+int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,
+                         const float *t0tx, const float *alatx, const float *doftx, int64_t nrx,
+                         const float *ovecrx, const float *nvecrx, const float *alarx,
+                         float *txpack, float *rxpack)
+{
+    if (ndim < 1 || ndim > 3) return fail(PYCBF_EINVAL, "ndim must be 1, 2 or 3");
+    if (ntx < 0 || nrx < 0) return fail(PYCBF_EINVAL, "negative element count");
+    for (int64_t e = 0; e < ntx; ++e) {
+        float *pk = txpack + e * PYCBF_TXPACK_FLOATS;
+        for (int i = 0; i < PYCBF_TXPACK_FLOATS; ++i) pk[i] = 0.0f;
+        for (int d = 0; d < ndim; ++d) {
+            pk[d] = ovectx[e * ndim + d];
+            pk[3 + d] = nvectx[e * ndim + d];
+        }
+        const float ala = alatx[e], dof = doftx[e];
+        pk[6] = t0tx[e];
+        pk[7] = (ala != 0.0f) ? ((dof != 0.0f) ? 2.0f : 1.0f) : 0.0f;
+        pk[8] = acos_threshold(ala);
+        const float sf = (float)sin((double)ala);
+        const float tf = (float)tan((double)ala);
+        pk[9] = dof * sf;
+        pk[10] = sqrtf(1.0f + tf * tf);
+    }
+    for (int64_t e = 0; e < nrx; ++e) {
+        float *pk = rxpack + e * PYCBF_RXPACK_FLOATS;
+        for (int i = 0; i < PYCBF_RXPACK_FLOATS; ++i) pk[i] = 0.0f;
+        for (int d = 0; d < ndim; ++d) {
+            pk[d] = ovecrx[e * ndim + d];
+            pk[3 + d] = nvecrx[e * ndim + d];
+        }
+        pk[6] = acos_threshold(alarx[e]);
+    }
+    return 0;
+}
+
+int pycbf_synthetic_tables(const pycbf_rfinfo *info, const float *txpack, const float *rxpack,
+                           float c0, const float *pvec, int64_t np, float *tautx, float *apodtx,
+                           float *taurx, float *apodrx, void *stream)
+{
+    if (!info || (info->ndim != 2 && info->ndim != 3))
+        return fail(PYCBF_EINVAL, "synthetic geometry needs ndim 2 or 3");
+    if (!txpack || !rxpack || !pvec || !tautx || !apodtx || !taurx || !apodrx)
+        return fail(PYCBF_EINVAL, "NULL device pointer");
+    if (info->ntx + info->nrx > 65535) return fail(PYCBF_EINVAL, "ntx + nrx must be <= 65535");
+    return launch_synth_tables(info->ndim, (int)info->ntx, (int)info->nrx, txpack, rxpack, c0,
+                               pvec, np, tautx, apodtx, taurx, apodrx, (hipStream_t)stream);
+}
+
+int pycbf_das_synthetic(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                        const float *txpack, const float *rxpack, float c0, const float *pvec,
+                        const int32_t *perm, int64_t np, int interp_kind, int interp_param,
+                        const float *S, float thresh, int sumflag, float *out, void *workspace,
+                        size_t workspace_bytes, void *stream)
+{
+    int rc = validate_common(info, rf, nfr, np, interp_kind, interp_param, S, sumflag, out);
+    if (rc) return rc;
+    if (info->ndim != 2 && info->ndim != 3)
+        return fail(PYCBF_EINVAL, "synthetic geometry needs ndim 2 or 3");
+    if (!txpack || !rxpack || !pvec) return fail(PYCBF_EINVAL, "NULL geometry pointer");
+    if (!(c0 > 0.0f) || !isfinite(c0)) return fail(PYCBF_EINVAL, "c0 must be finite and > 0");
+    const DasPlan plan = plan_das(info, nfr, np, interp_kind, interp_param, 1, sumflag);
+    if (workspace_bytes < plan.total || !workspace)
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
+                    workspace_bytes);
+    Carver cv(workspace, workspace_bytes);
+    const size_t tb = plan.tabs / 4;
+    float *ttx = (float *)cv.take(tb);
+    float *atx = (float *)cv.take(tb);
+    float *trx = (float *)cv.take(tb);
+    float *arx = (float *)cv.take(tb);
+    if (!ttx || !atx || !trx || !arx) return fail(PYCBF_EINVAL, "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    rc = pycbf_synthetic_tables(info, txpack, rxpack, c0, pvec, np, ttx, atx, trx, arx, stream);
+    if (rc) return rc;
+    return run_das(info, rf, nfr, ttx, atx, trx, arx, perm, np, interp_kind, interp_param, S,
+                   thresh, sumflag, out, plan, cv, s);
+}
+
+int pycbf_tile_order(const float *pvec, int64_t np, int ndim, int32_t *perm)
+{
+    if (!pvec || !perm || np < 0) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (ndim < 1 || ndim > 3) return fail(PYCBF_EINVAL, "ndim must be 1, 2 or 3");
+    if (np > 0x7fffffffLL) return fail(PYCBF_EINVAL, "np must fit in int32");
+    // grid coordinate per axis: dense rank of the value among the axis' unique
+    // values (exact for regular grids), quantised to 21 bits otherwise
+    std::vector<uint64_t> q((size_t)np * ndim);
+    for (int d = 0; d < ndim; ++d) {
+        std::vector<float> vals((size_t)np);
+        for (int64_t i = 0; i < np; ++i) vals[i] = pvec[i * ndim + d];
+        std::vector<float> u = vals;
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        const double scale = u.size() > (1u << 21) ? (double)((1u << 21) - 1) / (double)(u.size() - 1) : 1.0;
+        for (int64_t i = 0; i < np; ++i) {
+            const size_t r = (size_t)(std::lower_bound(u.begin(), u.end(), vals[i]) - u.begin());
+            q[i * ndim + d] = (uint64_t)((double)r * scale);
+        }
+    }
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)np);
+    for (int64_t i = 0; i < np; ++i) {
+        uint64_t code = 0;
+        for (int b = 0; b < 21; ++b)
+            for (int d = 0; d < ndim; ++d)
+                code |= ((q[i * ndim + d] >> b) & 1ull) << (b * ndim + d);
+        key[i] = {code, (int32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int64_t i = 0; i < np; ++i) perm[i] = key[i].second;
+    return 0;
+}
+
+size_t pycbf_spline_workspace_bytes(int64_t nch, int32_t nt)
+{
+    return spline_scratch_bytes(nch, nt) + 256;
+}
+
+int pycbf_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *coef,
+                       void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!rf || !coef || nch < 0 || nt < 1) return fail(PYCBF_EINVAL, "bad spline arguments");
+    Carver cv(workspace, workspace_bytes);
+    float *u = (float *)cv.take(spline_scratch_bytes(nch, nt));
+    if (!u) return fail(PYCBF_EINVAL, "workspace too small");
+    return launch_spline_coefs(rf, nch, nt, dt, u, (float4 *)coef, (hipStream_t)stream);
+}
+
+int pycbf_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *y2,
+                    void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!rf || !y2 || nch < 0 || nt < 1) return fail(PYCBF_EINVAL, "bad spline arguments");
+    Carver cv(workspace, workspace_bytes);
+    float *u = (float *)cv.take(spline_scratch_bytes(nch, nt));
+    if (!u) return fail(PYCBF_EINVAL, "workspace too small");
+    return launch_spline_y2(rf, nch, nt, dt, u, y2, (hipStream_t)stream);
+}
+
+int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const float *taurx,
+                        int64_t np, int interp_kind, int usf, int32_t *idx, void *stream)
+{
+    if (!info || !tautx || !taurx || !idx) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (info->ntx * info->nrx > 65535) return fail(PYCBF_EINVAL, "ntx*nrx must be <= 65535");
+    const int nt = info->tinfo.nx;
+    const bool nearest = interp_kind == PYCBF_INTERP_NEAREST;
+    if (nearest && usf < 1) return fail(PYCBF_EINVAL, "usf must be >= 1");
+    const int n = nearest ? usf * (nt - 1) + 1 : nt;
+    const float dt = nearest ? info->tinfo.dx / (float)usf : info->tinfo.dx;
+    return launch_debug_indices((int)info->ntx, (int)info->nrx, tautx, taurx, np, nearest ? 1 : 0,
+                                info->tinfo.x0, dt, n, idx, (hipStream_t)stream);
+}
+
+int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
+               int nlag, int dmastype, int sumlags, float *out, void *stream)
+{
+    if (!im || !lags || !out) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (nfr < 1 || nsep < 1 || np < 1 || nlag < 1) return fail(PYCBF_EINVAL, "empty DMAS problem");
+    if (dmastype < PYCBF_DMAS_SSR || dmastype > PYCBF_DMAS_PSSR)
+        return fail(PYCBF_EINVAL, "dmastype %d out of range", dmastype);
+    return launch_dmas(im, nfr, nsep, np, lags, nlag, dmastype, sumlags, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

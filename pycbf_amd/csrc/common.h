@@ -1,0 +1,90 @@
+// Shared declarations for the gfx950 beamforming library (libpycbf_hip.so).
+// All device code is compiled with -ffp-contract=off: every multiply-add that
+// must round like pycbf's float32 C code stays unfused, and the few places that
+// may fuse use __builtin_fmaf explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pycbf_hip.h"
+
+namespace pycbf {
+
+// Records a failure for pycbf_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...);
+// Turns a launch / runtime error into a status; 0 on success.
+int check(hipError_t e, const char *what);
+inline int check_launch(const char *what) { return check(hipGetLastError(), what); }
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of a caller-provided workspace.
+struct Carver {
+    char *base;
+    size_t cap;
+    size_t used = 0;
+    Carver(void *b, size_t c) : base(static_cast<char *>(b)), cap(c) {}
+    void *take(size_t bytes) {
+        size_t off = align_up(used, 256);
+        used = off + bytes;
+        return (base && used <= cap) ? base + off : nullptr;
+    }
+};
+
+// ---------------------------------------------------------------- spline ---
+constexpr int SPL_CHUNK = 32;  // samples per LDS tile in the spline sweeps
+// Sizes the spline scratch (one float per sample).
+inline size_t spline_scratch_bytes(int64_t nch, int32_t nt) {
+    return align_up((size_t)nch * (size_t)nt * sizeof(float), 256);
+}
+int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
+                        float4 *coef, hipStream_t s);
+int launch_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr, float *y2,
+                     hipStream_t s);
+int launch_upsample(const float *rf, const float *y2, int64_t nch, int32_t nt, float t0, float dt,
+                    int32_t usf, float *out, hipStream_t s);
+int launch_korder_coefs(const float *rf, int64_t nch, int32_t nt, const float *S, int k,
+                        float4 *coef, hipStream_t s);
+int launch_akima_coefs(const float *rf, int64_t nch, int32_t nt, int modified, float4 *coef,
+                       hipStream_t s);
+
+// ------------------------------------------------------------------- DAS ---
+// Gather kinds of the DAS kernel.
+//   GK_NEAREST  one sample of the raw or upsampled trace
+//   GK_LINEAR   two samples, weights a, b as the reference spline weights
+//   GK_CUBIC    natural spline, table (y_j, y_j+1, h*y2_j, h*y2_j+1) evaluated
+//               with the reference's separately rounded weights a and b
+//   GK_KORDER   Hermite table, t = ((x - t0) - dt*j)/dt  (korder_cubic rounding)
+//   GK_AKIMA    Hermite table, t = (x - (t0 + dt*j))/dt  (akima / makima rounding)
+enum GatherKind { GK_NEAREST = 0, GK_LINEAR = 1, GK_CUBIC = 2, GK_KORDER = 3, GK_AKIMA = 4 };
+
+struct DasArgs {
+    const float *ttx, *atx;  // (ntx, np)
+    const float *trx, *arx;  // (nrx, np)
+    const int32_t *perm;     // (np) or null
+    int64_t np;
+    const float *src;        // gather source (raw rf, upsampled rf, or float4 coefficients)
+    int64_t ch_stride;       // floats per channel in src
+    int ntx, nrx, nfr;
+    int nsrc;                // samples (or intervals) per channel
+    float t0, dt, rdt, tmax; // time axis of src
+    float thresh;
+    int sumflag;
+    int a_split;             // outer-axis elements per blockIdx.z
+    float *out;              // final output, or partial sums (nsplit, nfr, np)
+};
+
+int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
+int launch_synth_tables(int ndim, int ntx, int nrx, const float *txpack, const float *rxpack,
+                        float c0, const float *pvec, int64_t np, float *ttx, float *atx,
+                        float *trx, float *arx, hipStream_t s);
+int launch_debug_indices(int ntx, int nrx, const float *ttx, const float *trx, int64_t np,
+                         int nearest, float t0, float dt, int n, int32_t *idx, hipStream_t s);
+
+// ------------------------------------------------------------------ DMAS ---
+int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
+                int nlag, int dmastype, int sumlags, float *out, hipStream_t s);
+
+}  // namespace pycbf
