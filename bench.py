@@ -109,7 +109,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--cpu-stride", type=int, default=2)
+    ap.add_argument("--data", choices=["sim", "noise"], default="sim",
+                    help="noise: seeded Gaussian RF of the same shape (profiling runs)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one elementwise copy of this many MiB before timing "
+                         "(reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -126,7 +131,20 @@ def main():
     dev = torch.device("cuda", 0)
     xele, steers, pnts, params = workload_geometry()
     log(f"workload: {NTX} tx x {NRX} rx x {NX}x{NZ} px, nt={NT}, {args.frames} frames")
-    rf = make_rf(steers, xele, args.frames, dev)
+    if args.data == "sim":
+        rf = make_rf(steers, xele, args.frames, dev)
+    else:
+        g = torch.Generator(device=dev)
+        g.manual_seed(1234)
+        rf = torch.randn((args.frames, NTX, NRX, NT), generator=g, device=dev)
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"calibration copy: {args.calib_copy_mib} MiB read + written")
     bf = SyntheticBeamformer(**params, t0=0.0, dt=1.0 / 40e6, nt=NT, interp={"kind": "cubic"},
                              sumtype="tx_and_rx", device=dev)
     out = torch.empty((args.frames, NX * NZ), dtype=torch.float32, device=dev)

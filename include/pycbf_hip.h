@@ -134,9 +134,10 @@ int pycbf_synthetic_tables(const pycbf_rfinfo *info, const float *txpack, const 
 int pycbf_tile_order(const float *pvec, int64_t np, int ndim, int32_t *perm);
 
 /* Natural cubic spline of each channel (tie_knots1D_fixed, pycbf/cpu/pycbf.c:114-166)
- * as per-interval cubic coefficients: coef (nch, nt, 4) so that on interval j with
- * t = (x - t0 - j*dt)/dt the value is ((c3 t + c2) t + c1) t + c0.  The last
- * interval (j = nt-1) zero-pads the neighbour the reference reads out of range. */
+ * as the gather table coef (nch, nt+1, 2): entry k = (y_k, dt^2/6 * y2_k) with y2
+ * bit-identical to the reference, entry nt = (0, 0) (zero padding of the
+ * neighbour the reference reads out of range).  On interval j the value is
+ * a*y_j + b*y_j+1 + (a^3-a)*e_j + (b^3-b)*e_j+1 with e the second component. */
 int pycbf_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *coef,
                        void *workspace, size_t workspace_bytes, void *stream);
 

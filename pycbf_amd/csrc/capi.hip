@@ -3,6 +3,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <vector>
@@ -32,7 +33,7 @@ int check(hipError_t e, const char *what)
 struct DasPlan {
     int gk = GK_LINEAR;
     int64_t nch = 0;       // frames * ntx * nrx channels
-    size_t coef = 0;       // float4 coefficient table
+    size_t coef = 0;       // interpolation table (float2 or float4 per sample)
     size_t uscr = 0;       // spline sweep scratch
     size_t y2 = 0;         // spline second derivatives (nearest, usf > 1)
     size_t up = 0;         // upsampled channels (nearest, usf > 1)
@@ -42,24 +43,24 @@ struct DasPlan {
     size_t total = 0;
 };
 
-static int frames_per_pass_host(int64_t nfr)
+// Outer-axis groups of the gather grid.  One outer element per workgroup by
+// default: the grid is dispatched tile-fastest, so all resident workgroups
+// sweep the inner channel loop of the same outer element together and every
+// channel row is pulled through L2 about once (a channel wavefront).  For sums
+// over the outer axis each group writes a partial image; the group count is
+// capped so those partials stay within a memory budget.
+static int choose_split(int64_t np, int64_t nfr, int64_t na, bool partials)
 {
-    if (nfr >= 16) return 16;
-    if (nfr >= 8) return 8;
-    if (nfr >= 4) return 4;
-    if (nfr >= 2) return 2;
-    return 1;
-}
-
-static int choose_split(int64_t np, int64_t nfr, int64_t na)
-{
-    const int64_t blocks = ((np + 255) / 256) * ((nfr + frames_per_pass_host(nfr) - 1) /
-                                                  frames_per_pass_host(nfr));
-    const int64_t target = 2048;  // 8 workgroups per CU on 256 CUs
-    int64_t s = (target + blocks - 1) / blocks;
+    int64_t s = na;
+    if (partials) {
+        const int64_t budget = (int64_t)4 << 30;  // bytes of partial images
+        const int64_t per = nfr * np * (int64_t)sizeof(float);
+        const int64_t cap = per > 0 ? budget / per : na;
+        if (s > cap) s = cap;
+    }
+    if (const char *e = getenv("PYCBF_DAS_SPLIT")) s = atoll(e);
     if (s < 1) s = 1;
     if (s > na) s = na;
-    if (s > 64) s = 64;
     return (int)s;
 }
 
@@ -83,7 +84,7 @@ static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         break;
     case PYCBF_INTERP_CUBIC:
         p.gk = GK_CUBIC;
-        p.coef = align_up((size_t)p.nch * nt * sizeof(float4), 256);
+        p.coef = align_up((size_t)p.nch * (nt + 1) * sizeof(float2), 256);
         p.uscr = spline_scratch_bytes(p.nch, (int32_t)nt);
         break;
     case PYCBF_INTERP_KORDER_CUBIC:
@@ -100,11 +101,13 @@ static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
     // partial sums exist only for tx_and_rx (receivers outer); size them whenever
     // sumflag < 0 (workspace query without a sumtype) or tx_and_rx
     if (sumflag < 0 || sumflag == PYCBF_SUM_TX_AND_RX) {
-        p.nsplit = choose_split(np, nfr, info->nrx);
+        const int64_t na = das_outer_is_tx(PYCBF_SUM_TX_AND_RX, info->ntx, info->nrx)
+                               ? info->ntx : info->nrx;
+        p.nsplit = choose_split(np, nfr, na, true);
         if (p.nsplit > 1) p.part = align_up((size_t)p.nsplit * nfr * np * sizeof(float), 256);
     } else {
-        const int64_t na = (sumflag == PYCBF_SUM_RX_ONLY) ? info->ntx : info->nrx;
-        p.nsplit = choose_split(np, nfr, na);
+        const int64_t na = das_outer_is_tx(sumflag, info->ntx, info->nrx) ? info->ntx : info->nrx;
+        p.nsplit = choose_split(np, nfr, na, false);
     }
     p.total = p.coef + p.uscr + p.y2 + p.up + p.tabs + p.part + 1024;
     return p;
@@ -200,12 +203,12 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
     case PYCBF_INTERP_LINEAR:
         break;
     case PYCBF_INTERP_CUBIC: {
-        float4 *coef = (float4 *)cv.take(plan.coef);
+        float2 *coef = (float2 *)cv.take(plan.coef);
         float *u = (float *)cv.take(plan.uscr);
         if (!coef || !u) return fail(PYCBF_EINVAL, "workspace too small");
         if ((rc = launch_spline_coefs(rf, plan.nch, nt, dt, u, coef, s))) return rc;
         a.src = (const float *)coef;
-        a.ch_stride = 4 * (int64_t)nt;
+        a.ch_stride = 2 * ((int64_t)nt + 1);
         break;
     }
     case PYCBF_INTERP_KORDER_CUBIC: {
@@ -406,7 +409,7 @@ int pycbf_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float
     Carver cv(workspace, workspace_bytes);
     float *u = (float *)cv.take(spline_scratch_bytes(nch, nt));
     if (!u) return fail(PYCBF_EINVAL, "workspace too small");
-    return launch_spline_coefs(rf, nch, nt, dt, u, (float4 *)coef, (hipStream_t)stream);
+    return launch_spline_coefs(rf, nch, nt, dt, u, (float2 *)coef, (hipStream_t)stream);
 }
 
 int pycbf_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *y2,

@@ -39,8 +39,9 @@ constexpr int SPL_CHUNK = 32;  // samples per LDS tile in the spline sweeps
 inline size_t spline_scratch_bytes(int64_t nch, int32_t nt) {
     return align_up((size_t)nch * (size_t)nt * sizeof(float), 256);
 }
+// natural-spline table (nch, nt+1) of (y_k, dt^2/6 * y2_k), entry nt zero
 int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
-                        float4 *coef, hipStream_t s);
+                        float2 *coef, hipStream_t s);
 int launch_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr, float *y2,
                      hipStream_t s);
 int launch_upsample(const float *rf, const float *y2, int64_t nch, int32_t nt, float t0, float dt,
@@ -73,10 +74,16 @@ struct DasArgs {
     float thresh;
     int sumflag;
     int a_split;             // outer-axis elements per blockIdx.z
+    int xcd_remap;           // 1: contiguous tile ranges per XCD
+    int outer_tx;            // outer (per-block) axis is tx (else rx) for sums over both
     float *out;              // final output, or partial sums (nsplit, nfr, np)
 };
 
 int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
+// Frames handled per pass of the gather (1, 2, 4, 8 or 16; PYCBF_DAS_FC caps it).
+int das_frames_per_pass(int64_t nfr);
+// Whether transmits are the outer (one element per workgroup) axis.
+bool das_outer_is_tx(int sumflag, int64_t ntx, int64_t nrx);
 int launch_synth_tables(int ndim, int ntx, int nrx, const float *txpack, const float *rxpack,
                         float c0, const float *pvec, int64_t np, float *ttx, float *atx,
                         float *trx, float *arx, hipStream_t s);

@@ -12,6 +12,7 @@
 // pycbf/gpu/__engines__.cu:813-864): w = apodtx*apodrx, x = tautx+taurx; the
 // sample contributes w*interp(x) when |w| >= thresh and x lies on the trace.
 #include <math.h>
+#include <stdlib.h>
 
 #include "common.h"
 #include "prof.h"
@@ -43,6 +44,12 @@ __device__ __forceinline__ bool nearest_index(float x, float t0, float dt, int n
     it = (int)d;
     return true;
 }
+
+// Two consecutive (y, h*y2) table entries, loaded as one 16-byte access
+// from an 8-byte aligned address (gfx950 serves unaligned dwordx4 loads).
+struct __attribute__((aligned(8))) Pair2 {
+    float y0, z0, y1, z1;
+};
 
 // ---------------------------------------------------------------------------
 // gather + interpolate for FC frames of one channel
@@ -89,16 +96,16 @@ __device__ __forceinline__ void gather(const DasArgs &A, int64_t ch0, int64_t fs
         const float b = (x - xlag) * A.rdt;
         const float ca = __builtin_fmaf(a * a, a, -a);
         const float cb = __builtin_fmaf(b * b, b, -b);
-        const float4 *s = reinterpret_cast<const float4 *>(A.src) + ch0 * (A.ch_stride >> 2) + j;
-        const int64_t fs4 = fstride >> 2;
+        // entries j and j+1 of the (y, h*y2) table: one 16-byte load at 8-byte alignment
+        const float *s = A.src + ch0 * A.ch_stride + 2 * j;
 #pragma unroll
         for (int f = 0; f < FC; ++f)
             if (f < nf) {
-                const float4 c = s[f * fs4];
-                float r = a * c.x;
-                r = __builtin_fmaf(b, c.y, r);
-                r = __builtin_fmaf(ca, c.z, r);
-                v[f] = __builtin_fmaf(cb, c.w, r);
+                const Pair2 c = *reinterpret_cast<const Pair2 *>(s + f * fstride);
+                float r = a * c.y0;
+                r = __builtin_fmaf(b, c.y1, r);
+                r = __builtin_fmaf(ca, c.z0, r);
+                v[f] = __builtin_fmaf(cb, c.z1, r);
             }
     } else {
         int j = 0;
@@ -117,26 +124,47 @@ __device__ __forceinline__ void gather(const DasArgs &A, int64_t ch0, int64_t fs
     }
 }
 
+// Workgroups go to the 8 XCDs round robin by linear id.  With remap on, the
+// linear id is permuted so each XCD receives one contiguous run of pixel
+// tiles: tiles that are neighbours in the (spatially compact) processing order
+// read overlapping sample windows of every channel, and now share an L2.
+__device__ __forceinline__ void block_coords(int remap, uint32_t &bx, uint32_t &by, uint32_t &bz)
+{
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    uint32_t lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (remap) {
+        const uint32_t per = (gx * gy * gridDim.z) >> 3;
+        if (lin < (per << 3)) lin = (lin & 7u) * per + (lin >> 3);
+    }
+    bx = lin % gx;
+    const uint32_t r = lin / gx;
+    by = r % gy;
+    bz = r / gy;
+}
+
 template <int GK, int SUM, int FC>
 __global__ __launch_bounds__(256) void k_das(DasArgs A)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bx, by, bz;
+    block_coords(A.xcd_remap, bx, by, bz);
+    const int64_t i = (int64_t)bx * blockDim.x + threadIdx.x;
     if (i >= A.np) return;
     const int64_t p = A.perm ? (int64_t)A.perm[i] : i;
-    const int f0 = blockIdx.y * FC;
+    const int f0 = by * FC;
     const int nf = min(FC, A.nfr - f0);
     const int64_t np = A.np;
 
-    // rx_only keeps transmits apart, so transmits are the outer axis there;
-    // everywhere else receivers are outer and transmits inner.
-    constexpr bool OUTER_TX = (SUM == PYCBF_SUM_RX_ONLY);
+    // rx_only keeps transmits apart and tx_only receivers, so those are the
+    // outer axis there; tx_and_rx and none take the axis the host chose.
+    const bool OUTER_TX = (SUM == PYCBF_SUM_RX_ONLY) ||
+                          (SUM != PYCBF_SUM_TX_ONLY && A.outer_tx);
     const int na = OUTER_TX ? A.ntx : A.nrx;
     const int nb = OUTER_TX ? A.nrx : A.ntx;
     const float *ta_t = OUTER_TX ? A.ttx : A.trx;
     const float *wa_t = OUTER_TX ? A.atx : A.arx;
     const float *tb_t = OUTER_TX ? A.trx : A.ttx;
     const float *wb_t = OUTER_TX ? A.arx : A.atx;
-    const int a0 = blockIdx.z * A.a_split;
+    const int a0 = bz * A.a_split;
     const int a1 = min(na, a0 + A.a_split);
 
     const int64_t nchan = (int64_t)A.ntx * A.nrx;
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
     }
     if (SUM == PYCBF_SUM_TX_AND_RX) {
         // gridDim.z > 1: partial sums (z, nfr, np) reduced by k_reduce_split
-        float *o = A.out + (int64_t)blockIdx.z * A.nfr * np;
+        float *o = A.out + (int64_t)bz * A.nfr * np;
 #pragma unroll
         for (int f = 0; f < FC; ++f)
             if (f < nf) o[(int64_t)(f0 + f) * np + p] = acc[f];
@@ -227,21 +255,30 @@ static void launch_sum(int sum, int fc, const DasArgs &a, dim3 grid, hipStream_t
     }
 }
 
-static int frames_per_pass(int nfr)
+bool das_outer_is_tx(int sumflag, int64_t ntx, int64_t nrx)
 {
-    if (nfr >= 16) return 16;
-    if (nfr >= 8) return 8;
-    if (nfr >= 4) return 4;
-    if (nfr >= 2) return 2;
-    return 1;
+    if (sumflag == PYCBF_SUM_RX_ONLY) return true;
+    if (sumflag == PYCBF_SUM_TX_ONLY) return false;
+    return ntx <= nrx;  // fewer outer elements: fewer partial images to reduce
+}
+
+int das_frames_per_pass(int64_t nfr)
+{
+    int cap = 16;
+    if (const char *e = getenv("PYCBF_DAS_FC")) cap = atoi(e);
+    int fc = 1;
+    while (fc * 2 <= cap && fc * 2 <= 16 && fc * 2 <= nfr) fc *= 2;
+    return fc;
 }
 
 int launch_das(int gk, const DasArgs &a0, int nsplit, float *partial, hipStream_t s)
 {
     DasArgs a = a0;
-    const int fc = frames_per_pass(a.nfr);
-    const bool outer_tx = (a.sumflag == PYCBF_SUM_RX_ONLY);
-    const int na = outer_tx ? a.ntx : a.nrx;
+    const int fc = das_frames_per_pass(a.nfr);
+    a.xcd_remap = 0;
+    if (const char *e = getenv("PYCBF_XCD_REMAP")) a.xcd_remap = atoi(e);
+    const int na = das_outer_is_tx(a.sumflag, a.ntx, a.nrx) ? a.ntx : a.nrx;
+    a.outer_tx = das_outer_is_tx(a.sumflag, a.ntx, a.nrx) ? 1 : 0;
     if (nsplit < 1) nsplit = 1;
     if (nsplit > na) nsplit = na;
     a.a_split = (na + nsplit - 1) / nsplit;

@@ -20,139 +20,182 @@
 
 namespace pycbf {
 
-// First 32 data-independent forward coefficients of the spline sweep,
-// y2f[i] = (0.5 - 1) / (0.5*y2f[i-1] + 2).  In float32 the sequence reaches an
-// exact fixed point at i = 7, so y2f[i] == y2f[31] for every i >= 31.
-__device__ __forceinline__ float y2f_at(const float *tab, int i) { return tab[i < 31 ? i : 31]; }
+// The data-independent forward coefficients of the spline sweep,
+// y2f[i] = (0.5 - 1) / (0.5*y2f[i-1] + 2), reach an exact float32 fixed point
+// at i = 7; the kernels keep the first 32 and use y2f[31] for every later i.
 
-template <int MODE>  // 0: float4 cubic coefficients, 1: float y2
+// 16 floats of a lane-private row, read or written as four 16-byte accesses
+// (rows need only 4-byte alignment; gfx950 serves unaligned dwordx4 accesses).
+struct __attribute__((aligned(4))) F4u {
+    float v[4];
+};
+
+template <int N>
+__device__ __forceinline__ void row_load(const float *row, int s, int nt, float (&d)[N])
+{
+    if (s + N <= nt) {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            const F4u w = *reinterpret_cast<const F4u *>(row + s + 4 * q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) d[4 * q + e] = w.v[e];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < N; ++c) d[c] = (s + c < nt) ? row[s + c] : 0.0f;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void row_store(float *row, int s, int nt, const float (&d)[N])
+{
+    if (s + N <= nt) {
+#pragma unroll
+        for (int q = 0; q < N / 4; ++q) {
+            F4u w;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) w.v[e] = d[4 * q + e];
+            *reinterpret_cast<F4u *>(row + s + 4 * q) = w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            if (s + c < nt) row[s + c] = d[c];
+    }
+}
+
+// One lane per channel.  Each lane streams its own row in 32-sample chunks
+// with the next chunk prefetched into registers.  Per chunk the data-parallel
+// part of the forward sweep, g_i = 6*((y[i+1]-y[i])/dx - (y[i]-y[i-1])/dx)/(2*dx),
+// is computed first (32 independent lanes of work), so only
+// u_i = (g_i - 0.5*u_{i-1}) / p_i stays on the serial dependency chain.  The
+// operations and their order are exactly those of tie_knots1D_fixed
+// (pycbf/cpu/pycbf.c:126-141), so y2 is bit-identical.
+template <int MODE>  // 0: (y, dx^2/6*y2) gather table, 1: y2
 __global__ __launch_bounds__(64) void k_spline(const float *__restrict__ rf, int64_t nch, int nt,
                                                float dx, float *__restrict__ uscr,
                                                void *__restrict__ outp)
 {
     constexpr int CH = SPL_CHUNK;
-    __shared__ float ytile[64][CH + 1];  // column CH holds the next tile's first sample
-    __shared__ float utile[64][CH + 1];
-    __shared__ float y2f[32];
+    const int64_t ch = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (ch >= nch) return;
 
-    const int lane = threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.x * 64;
-    const int64_t ch = c0 + lane;
-    const bool active = ch < nch;
-    const int nrow = (int)((nch - c0) < 64 ? (nch - c0) : 64);
-
-    if (lane == 0) {
+    float y2f[32];
+    {
         float v = 0.0f;
         y2f[0] = 0.0f;
+#pragma unroll
         for (int i = 1; i < 32; ++i) {
-            float p = 0.5f * v + 2.0f;
+            const float p = 0.5f * v + 2.0f;
             v = (0.5f - 1.0f) / p;
             y2f[i] = v;
         }
     }
-    __syncthreads();
+    // p_i = 0.5*y2f[i-1] + 2 for i >= 32 (fixed point)
+    const float p_inf = 0.5f * y2f[31] + 2.0f;
 
+    const float *row = rf + ch * (int64_t)nt;
+    float *urow = uscr + ch * (int64_t)nt;
     const int nchunk = (nt + CH - 1) / CH;
 
-    // ---- forward sweep: u[i] -------------------------------------------
-    float u_prev = 0.0f, y_prev = 0.0f;
+    // ---- forward sweep ---------------------------------------------------
+    float ycur[CH], ynext[CH];
+    row_load<CH>(row, 0, nt, ycur);
+    float y_prev = 0.0f, u_prev = 0.0f;
     for (int cb = 0; cb < nchunk; ++cb) {
         const int s = cb * CH;
-#pragma unroll 8
-        for (int r0 = 0; r0 < 64; r0 += 2) {
-            const int r = r0 + (lane >> 5), col = lane & 31;
-            const int t = s + col;
-            float v = 0.0f;
-            if (r < nrow && t < nt) v = rf[(c0 + r) * (int64_t)nt + t];
-            ytile[r][col] = v;
+        if (cb + 1 < nchunk) row_load<CH>(row, s + CH, nt, ynext);
+        const float y_after = (cb + 1 < nchunk) ? ynext[0] : 0.0f;
+        float g[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const float yp = (c == 0) ? y_prev : ycur[c - 1];
+            const float yn = (c == CH - 1) ? y_after : ycur[c + 1];
+            const float jump = (yn - ycur[c]) / dx - (ycur[c] - yp) / dx;
+            g[c] = 6.0f * jump / (2.0f * dx);
         }
-        {
-            const int t = s + CH;
-            float v = 0.0f;
-            if (active && t < nt) v = rf[ch * (int64_t)nt + t];
-            ytile[lane][CH] = v;
-        }
-        __syncthreads();
-#pragma unroll 2
-        for (int col = 0; col < CH; ++col) {
-            const int i = s + col;
-            const float yi = ytile[lane][col];
-            float u = 0.0f;
-            if (i >= 1 && i <= nt - 2) {
-                const float yn = ytile[lane][col + 1];
-                const float p = 0.5f * y2f_at(y2f, i - 1) + 2.0f;
-                const float jump = (yn - yi) / dx - (yi - y_prev) / dx;
-                u = (6.0f * jump / (2.0f * dx) - 0.5f * u_prev) / p;
-                u_prev = u;
+        float u[CH];
+        if (s >= 32 && s + CH <= nt - 1) {
+            // interior chunk: p is the fixed point, every i in [1, nt-2]
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                u_prev = (g[c] - 0.5f * u_prev) / p_inf;
+                u[c] = u_prev;
             }
-            y_prev = yi;
-            utile[lane][col] = u;
+        } else {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int i = s + c;
+                float ui = 0.0f;
+                if (i >= 1 && i <= nt - 2) {
+                    const float p = 0.5f * y2f[(i - 1) < 31 ? (i - 1) : 31] + 2.0f;
+                    ui = (g[c] - 0.5f * u_prev) / p;
+                    u_prev = ui;
+                }
+                u[c] = ui;
+            }
         }
-        __syncthreads();
-#pragma unroll 8
-        for (int r0 = 0; r0 < 64; r0 += 2) {
-            const int r = r0 + (lane >> 5), col = lane & 31;
-            const int t = s + col;
-            if (r < nrow && t < nt) uscr[(c0 + r) * (int64_t)nt + t] = utile[r][col];
-        }
-        __syncthreads();
+        row_store<CH>(urow, s, nt, u);
+        y_prev = ycur[CH - 1];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) ycur[c] = ynext[c];
     }
 
-    // ---- back substitution: y2[k] = y2f[k]*y2[k+1] + u[k] ----------------
+    // ---- back substitution: y2[k] = y2f[k]*y2[k+1] + u[k] ------------------
     const float h = (1.0f / 6.0f) * (dx * dx);
-    float y2_next = 0.0f;  // y2[k+1]; zero beyond the end (natural end + padding)
-    float y_next = 0.0f;   // y[k+1];  zero beyond the end (padding)
+    float y2_next = 0.0f;  // y2[k+1]; zero at the natural end and beyond
+    float ucur[CH], unext[CH];
+    const int slast = (nchunk - 1) * CH;
+    row_load<CH>(row, slast, nt, ycur);
+    row_load<CH>(urow, slast, nt, ucur);
+    float2 *tab = static_cast<float2 *>(outp) + ch * (int64_t)(nt + 1);
+    float *y2row = static_cast<float *>(outp) + ch * (int64_t)nt;
+    if (MODE == 0) tab[nt] = make_float2(0.0f, 0.0f);
     for (int cb = nchunk - 1; cb >= 0; --cb) {
         const int s = cb * CH;
-#pragma unroll 8
-        for (int r0 = 0; r0 < 64; r0 += 2) {
-            const int r = r0 + (lane >> 5), col = lane & 31;
-            const int t = s + col;
-            float v = 0.0f, u = 0.0f;
-            if (r < nrow && t < nt) {
-                v = rf[(c0 + r) * (int64_t)nt + t];
-                u = uscr[(c0 + r) * (int64_t)nt + t];
-            }
-            ytile[r][col] = v;
-            utile[r][col] = u;
+        if (cb > 0) {
+            row_load<CH>(row, s - CH, nt, ynext);
+            row_load<CH>(urow, s - CH, nt, unext);
         }
-        __syncthreads();
-        float4 *coef = static_cast<float4 *>(outp);
-#pragma unroll 2
-        for (int col = CH - 1; col >= 0; --col) {
-            const int k = s + col;
-            if (k < nt) {
-                const float yk = ytile[lane][col];
-                float y2k = 0.0f;
-                if (k < nt - 1) y2k = y2f_at(y2f, k) * y2_next + utile[lane][col];
-                if (MODE == 1) {
-                    utile[lane][col] = y2k;
-                } else if (active) {
-                    // interval k: value = a*y_k + b*y_k+1 + (a^3-a)*h*y2_k
-                    // + (b^3-b)*h*y2_k+1 with h = dx^2/6 (pycbf/cpu/pycbf.c:97-103).
-                    // Lane-private rows: eight consecutive intervals complete
-                    // each 128-byte line, which L2 merges before write-back.
-                    coef[ch * (int64_t)nt + k] = make_float4(yk, y_next, h * y2k, h * y2_next);
+        float out[CH];
+#pragma unroll
+        for (int c = CH - 1; c >= 0; --c) {
+            const int k = s + c;
+            float y2k = 0.0f;
+            if (k < nt - 1) y2k = y2f[k < 31 ? k : 31] * y2_next + ucur[c];
+            if (k < nt) y2_next = y2k;
+            out[c] = (MODE == 0) ? h * y2k : y2k;
+        }
+        if (MODE == 0) {
+            // entries (y_k, h*y2_k), two per 16-byte store
+            if (s + CH <= nt) {
+#pragma unroll
+                for (int q = 0; q < CH / 2; ++q) {
+                    F4u w;
+                    w.v[0] = ycur[2 * q];
+                    w.v[1] = out[2 * q];
+                    w.v[2] = ycur[2 * q + 1];
+                    w.v[3] = out[2 * q + 1];
+                    *reinterpret_cast<F4u *>(tab + s + 2 * q) = w;
                 }
-                y2_next = y2k;
-                y_next = yk;
+            } else {
+#pragma unroll
+                for (int c = 0; c < CH; ++c)
+                    if (s + c < nt) tab[s + c] = make_float2(ycur[c], out[c]);
             }
+        } else {
+            row_store<CH>(y2row, s, nt, out);
         }
-        if (MODE == 1) {
-            __syncthreads();
-            float *y2 = static_cast<float *>(outp);
-            for (int r0 = 0; r0 < 64; r0 += 2) {
-                const int r = r0 + (lane >> 5), col = lane & 31;
-                const int t = s + col;
-                if (r < nrow && t < nt) y2[(c0 + r) * (int64_t)nt + t] = utile[r][col];
-            }
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            ycur[c] = ynext[c];
+            ucur[c] = unext[c];
         }
-        __syncthreads();
     }
 }
 
 int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
-                        float4 *coef, hipStream_t s)
+                        float2 *coef, hipStream_t s)
 {
     if (nch <= 0 || nt <= 0) return 0;
     dim3 grid((unsigned)((nch + 63) / 64));

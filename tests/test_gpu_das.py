@@ -74,6 +74,35 @@ def test_index_selection_bitwise(c0, kind, usf):
     assert (ref >= 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("nt", [1, 2, 3, 31, 33, 64, 100, 2048])
+def test_spline_prefilter_bitwise(nt):
+    """Second derivatives (pycbf_spline_y2) and the gather table
+    (pycbf_spline_coefs) are bit-identical to tie_knots1D_fixed's."""
+    import torch
+    from pycbf_amd.gpu import _engine as E
+    rng = np.random.default_rng(nt)
+    nch, dt = 70, np.float32(2.5e-8)
+    rf = rng.standard_normal((nch, nt)).astype(np.float32)
+    dev = torch.device("cuda")
+    x = torch.as_tensor(rf, device=dev)
+    wsb = E.lib.pycbf_spline_workspace_bytes(nch, nt)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    y2 = torch.empty((nch, nt), dtype=torch.float32, device=dev)
+    tab = torch.empty((nch, nt + 1, 2), dtype=torch.float32, device=dev)
+    s = E.stream_of(dev)
+    E.check(E.lib.pycbf_spline_y2(E.ptr(x), nch, nt, float(dt), E.ptr(y2), E.ptr(ws), wsb, s), "y2")
+    E.check(E.lib.pycbf_spline_coefs(E.ptr(x), nch, nt, float(dt), E.ptr(tab), E.ptr(ws), wsb, s),
+            "coefs")
+    ref = np.stack([oracle.spline_y2(r, float(dt)) for r in rf])
+    got = y2.cpu().numpy()
+    assert np.array_equal(got, ref), f"{int((got != ref).sum())} y2 values differ"
+    h = np.float32(np.float32(1.0) / np.float32(6.0)) * (dt * dt)
+    t = tab.cpu().numpy()
+    assert np.array_equal(t[:, :nt, 0], rf)
+    assert np.array_equal(t[:, :nt, 1], (h * ref).astype(np.float32))
+    assert np.all(t[:, nt] == 0)
+
+
 @pytest.mark.parametrize("sumtype", ["none", "tx_only", "rx_only", "tx_and_rx"])
 @pytest.mark.parametrize("interp,kind,usf", KINDS, ids=[k[1] + str(k[2]) for k in KINDS])
 def test_sumtypes_match_oracle(small, sumtype, interp, kind, usf):
