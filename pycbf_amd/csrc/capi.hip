@@ -23,6 +23,12 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
+bool das_use_lds()
+{
+    const char *e = getenv("PYCBF_DAS_LDS");
+    return !(e && atoi(e) == 0);
+}
+
 int check(hipError_t e, const char *what)
 {
     if (e == hipSuccess) return 0;
@@ -39,6 +45,7 @@ struct DasPlan {
     size_t up = 0;         // upsampled channels (nearest, usf > 1)
     size_t tabs = 0;       // synthetic delay / apodisation tables
     size_t part = 0;       // split partial sums
+    size_t bounds = 0;     // per-tile delay ranges (staged gather)
     int nsplit = 1;
     size_t total = 0;
 };
@@ -96,7 +103,9 @@ static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         p.coef = align_up((size_t)p.nch * nt * sizeof(float4), 256);
         break;
     }
-    if (synthetic)
+    if (das_use_lds())
+        p.bounds = align_up((size_t)das_tiles(np) * (info->ntx + info->nrx) * 2 * sizeof(float), 256);
+    else if (synthetic)
         p.tabs = 4 * align_up((size_t)(info->ntx + info->nrx) * np * sizeof(float), 256);
     // partial sums exist only for tx_and_rx (receivers outer); size them whenever
     // sumflag < 0 (workspace query without a sumtype) or tx_and_rx
@@ -109,7 +118,7 @@ static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         const int64_t na = das_outer_is_tx(sumflag, info->ntx, info->nrx) ? info->ntx : info->nrx;
         p.nsplit = choose_split(np, nfr, na, false);
     }
-    p.total = p.coef + p.uscr + p.y2 + p.up + p.tabs + p.part + 1024;
+    p.total = p.coef + p.uscr + p.y2 + p.up + p.tabs + p.part + p.bounds + 2048;
     return p;
 }
 
@@ -124,6 +133,7 @@ static int validate_common(const pycbf_rfinfo *info, const float *rf, int64_t nf
     if (nfr < 1 || np < 1) return fail(PYCBF_EINVAL, "nfr and np must be >= 1");
     if ((info->ntx * info->nrx) > 0x7fffffffLL)
         return fail(PYCBF_EINVAL, "ntx*nrx must fit in int32");
+    if (info->ntx + info->nrx > 65535) return fail(PYCBF_EINVAL, "ntx + nrx must be <= 65535");
     if (!rf || !out) return fail(PYCBF_EINVAL, "rf and out must be device pointers");
     if (sumflag < PYCBF_SUM_NONE || sumflag > PYCBF_SUM_TX_AND_RX)
         return fail(PYCBF_EINVAL, "sumflag %d out of range", sumflag);
@@ -155,10 +165,17 @@ static int validate_common(const pycbf_rfinfo *info, const float *rf, int64_t nf
 }
 
 // Builds the gather source for `kind` into the workspace and runs the gather.
+struct SynGeo {
+    const float *txpack = nullptr, *rxpack = nullptr, *pvec = nullptr;
+    float c0 = 0.0f;
+    int ndim = 0;  // 0: tabbed
+};
+
 static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const float *ttx,
                    const float *atx, const float *trx, const float *arx, const int32_t *perm,
                    int64_t np, int kind, int param, const float *S, float thresh, int sumflag,
-                   float *out, const DasPlan &plan, Carver &cv, hipStream_t s)
+                   float *out, const DasPlan &plan, Carver &cv, hipStream_t s,
+                   const SynGeo &geo = SynGeo())
 {
     const int nt = info->tinfo.nx;
     const float t0 = info->tinfo.x0, dt = info->tinfo.dx;
@@ -182,6 +199,12 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
     a.src = rf;
     a.ch_stride = nt;
     a.nsrc = nt;
+    a.nent = nt;
+    a.geo = geo.ndim;
+    a.txpack = geo.txpack;
+    a.rxpack = geo.rxpack;
+    a.pvec = geo.pvec;
+    a.c0 = geo.c0;
     int rc = 0;
     switch (kind) {
     case PYCBF_INTERP_NEAREST:
@@ -196,6 +219,7 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
             a.src = up;
             a.ch_stride = ntu;
             a.nsrc = ntu;
+            a.nent = ntu;
             a.dt = dt / (float)param;
             a.rdt = 1.0f / a.dt;
         }
@@ -209,6 +233,7 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
         if ((rc = launch_spline_coefs(rf, plan.nch, nt, dt, u, coef, s))) return rc;
         a.src = (const float *)coef;
         a.ch_stride = 2 * ((int64_t)nt + 1);
+        a.nent = nt + 1;
         break;
     }
     case PYCBF_INTERP_KORDER_CUBIC: {
@@ -233,6 +258,13 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
     if (plan.part) {
         part = (float *)cv.take(plan.part);
         if (!part) return fail(PYCBF_EINVAL, "workspace too small");
+    }
+    if (plan.bounds) {
+        float *bnd = (float *)cv.take(plan.bounds);
+        if (!bnd) return fail(PYCBF_EINVAL, "workspace too small");
+        if ((rc = launch_tile_bounds(a, bnd, s))) return rc;
+        a.bounds = bnd;
+        return launch_das_lds(plan.gk, a, plan.nsplit, part, s);
     }
     return launch_das(plan.gk, a, plan.nsplit, part, s);
 }
@@ -351,13 +383,24 @@ int pycbf_das_synthetic(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
                     workspace_bytes);
     Carver cv(workspace, workspace_bytes);
+    hipStream_t s = (hipStream_t)stream;
+    if (plan.bounds) {
+        // staged gather: delays and weights are evaluated in registers
+        SynGeo g;
+        g.txpack = txpack;
+        g.rxpack = rxpack;
+        g.pvec = pvec;
+        g.c0 = c0;
+        g.ndim = info->ndim;
+        return run_das(info, rf, nfr, nullptr, nullptr, nullptr, nullptr, perm, np, interp_kind,
+                       interp_param, S, thresh, sumflag, out, plan, cv, s, g);
+    }
     const size_t tb = plan.tabs / 4;
     float *ttx = (float *)cv.take(tb);
     float *atx = (float *)cv.take(tb);
     float *trx = (float *)cv.take(tb);
     float *arx = (float *)cv.take(tb);
     if (!ttx || !atx || !trx || !arx) return fail(PYCBF_EINVAL, "workspace too small");
-    hipStream_t s = (hipStream_t)stream;
     rc = pycbf_synthetic_tables(info, txpack, rxpack, c0, pvec, np, ttx, atx, trx, arx, stream);
     if (rc) return rc;
     return run_das(info, rf, nfr, ttx, atx, trx, arx, perm, np, interp_kind, interp_param, S,

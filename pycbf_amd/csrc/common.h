@@ -77,9 +77,25 @@ struct DasArgs {
     int xcd_remap;           // 1: contiguous tile ranges per XCD
     int outer_tx;            // outer (per-block) axis is tx (else rx) for sums over both
     float *out;              // final output, or partial sums (nsplit, nfr, np)
+    // LDS-staged gather only
+    int geo;                 // 0: delay tables; 2 / 3: synthetic geometry of that ndim
+    const float *txpack, *rxpack, *pvec;  // synthetic geometry (geo > 0)
+    float c0;
+    const float *bounds;     // (ntiles, ntx + nrx, 2) delay min/max per pixel tile
+    int nent;                // table entries per channel row of src
 };
 
 int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
+// LDS-staged gather (delay tables or in-register synthetic geometry); bounds
+// must hold the tile delay ranges written by launch_tile_bounds.
+int launch_das_lds(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
+// Delay min/max per (pixel tile of 256, element): tx elements then rx.
+int launch_tile_bounds(const DasArgs &a, float *bounds, hipStream_t s);
+inline int64_t das_tiles(int64_t np) { return (np + 255) / 256; }
+// out[g] = sum over z < nsplit of part[z*n + g]
+int launch_reduce_split(const float *part, int nsplit, int64_t n, float *out, hipStream_t s);
+// Whether the LDS-staged gather is used (PYCBF_DAS_LDS=0 selects the direct one).
+bool das_use_lds();
 // Frames handled per pass of the gather (1, 2, 4, 8 or 16; PYCBF_DAS_FC caps it).
 int das_frames_per_pass(int64_t nfr);
 // Whether transmits are the outer (one element per workgroup) axis.

@@ -1,0 +1,197 @@
+// Device helpers shared by the DAS kernels: sample-index selection, the
+// synthetic-point geometry and workgroup-coordinate mapping.
+#pragma once
+#include <math.h>
+
+#include "common.h"
+
+namespace pycbf {
+
+// Knot index of pycbf/cpu/pycbf.c:75-87: fill outside [t0, tmax] (NaN too).
+// The clamp to the last interval only matters when t0 is so large that tmax
+// rounds more than one sample past the end (the reference then reads out of
+// bounds); for every other input the truncated quotient is already <= n-1.
+__device__ __forceinline__ bool knot_index(float x, float t0, float dt, float tmax, int n, int &j)
+{
+    if (!(x >= t0 && x <= tmax)) return false;
+    j = min((int)((x - t0) / dt), n - 1);
+    return true;
+}
+
+// Nearest index of pycbf/cpu/pycbf.c:264,267: (int)(q + 0.5) with the 0.5 added
+// in double; valid when 0 <= it < n.
+__device__ __forceinline__ bool nearest_index(float x, float t0, float dt, int n, int &it)
+{
+    const float q = (x - t0) / dt;
+    const double d = (double)q + 0.5;
+    if (!(d > -1.0 && d < (double)n)) return false;
+    it = (int)d;
+    return true;
+}
+
+// Two consecutive (y, h*y2) table entries, loaded as one 16-byte access
+// from an 8-byte aligned address (gfx950 serves unaligned dwordx4 loads).
+struct __attribute__((aligned(8))) Pair2 {
+    float y0, z0, y1, z1;
+};
+
+// Workgroups go to the 8 XCDs round robin by linear id.  With remap on, the
+// linear id is permuted so each XCD receives one contiguous run of pixel
+// tiles: tiles that are neighbours in the (spatially compact) processing order
+// read overlapping sample windows of every channel, and now share an L2.
+__device__ __forceinline__ void block_coords(int remap, uint32_t &bx, uint32_t &by, uint32_t &bz)
+{
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    uint32_t lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (remap) {
+        const uint32_t per = (gx * gy * gridDim.z) >> 3;
+        if (lin < (per << 3)) lin = (lin & 7u) * per + (lin >> 3);
+    }
+    bx = lin % gx;
+    const uint32_t r = lin / gx;
+    by = r % gy;
+    bz = r / gy;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic-point geometry (pycbf/gpu/__engines__.cu:382-470), float32, unfused
+// ---------------------------------------------------------------------------
+
+// "acos(v) > ala" for the T of the element's pack; a NaN acos (|v| > 1) is false.
+__device__ __forceinline__ bool acos_gt(float v, float T) { return (v >= -1.0f && v <= 1.0f) && (v < T); }
+// "acos(v) <= ala"
+__device__ __forceinline__ bool acos_le(float v, float T) { return (v >= -1.0f && v <= 1.0f) && !(v < T); }
+
+template <int NDIM>
+__device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, const float *p,
+                                        float &tau, float &apod)
+{
+    float mag = 0.0f, proj = 0.0f;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        const float di = p[d] - pk[d];
+        mag += di * di;
+        proj += di * pk[3 + d];
+    }
+    mag = sqrtf(mag);
+    const float t0 = pk[6];
+    const int mode = (int)pk[7];
+    const float T = pk[8];
+    tau = 0.0f;
+    if (mode == 0) {  // plane wave
+        tau = proj / c0;
+        apod = 1.0f;
+        return;
+    }
+    const float sgn = (proj > 0.0f) ? 1.0f : -1.0f;
+    if (mode == 2) {  // focus with depth of field
+        const bool in_cone = ((double)fabsf(proj) > 1e-12) && ((double)mag > 1e-12) &&
+                             acos_le(fabsf(proj / mag), T);
+        if (in_cone) {
+            tau = sgn * (mag / c0) + t0;
+            apod = 1.0f;
+        } else {
+            const float lat = sqrtf(fabsf(mag * mag - proj * proj));
+            if ((double)lat < (double)pk[9] / 2.0) {
+                tau = proj * pk[10] / c0 + t0;
+                apod = 1.0f;
+            } else {
+                apod = 0.0f;
+            }
+        }
+        return;
+    }
+    tau = ((double)fabsf(proj) > 1e-12) ? sgn * (mag / c0) + t0 : t0;
+    apod = (((double)mag > 1e-12) && acos_gt(fabsf(proj / mag), T)) ? 0.0f : 1.0f;
+}
+
+template <int NDIM>
+__device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, const float *p,
+                                        float &tau, float &apod)
+{
+    float mag = 0.0f, proj = 0.0f;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+        const float di = p[d] - pk[d];
+        mag += di * di;
+        proj += di * pk[3 + d];
+    }
+    mag = sqrtf(mag);
+    tau = mag / c0;
+    apod = (((double)mag > 1e-12) && acos_gt(proj / mag, pk[6])) ? 0.0f : 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// gather + interpolate for FC frames of one channel
+// ---------------------------------------------------------------------------
+template <int GK, int FC>
+__device__ __forceinline__ void gather(const DasArgs &A, int64_t ch0, int64_t fstride, int nf,
+                                       float x, bool &ok, float (&v)[FC])
+{
+    if (GK == GK_NEAREST) {
+        int it = 0;
+        ok = nearest_index(x, A.t0, A.dt, A.nsrc, it);
+        if (!ok) return;
+        const float *s = A.src + ch0 * A.ch_stride + it;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) v[f] = s[f * fstride];
+    } else if (GK == GK_LINEAR) {
+        int j = 0;
+        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+        if (!ok) return;
+        const float xlead = A.t0 + A.dt * (float)(j + 1);
+        const float xlag = A.t0 + A.dt * (float)j;
+        const float a = (xlead - x) / A.dt;
+        const float b = (x - xlag) / A.dt;
+        const bool hi = (j + 1) < A.nsrc;
+        const float *s = A.src + ch0 * A.ch_stride + j;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) {
+                const float y0 = s[f * fstride];
+                const float y1 = hi ? s[f * fstride + 1] : 0.0f;
+                v[f] = a * y0 + b * y1;
+            }
+    } else if (GK == GK_CUBIC) {
+        // value = a*y_j + b*y_j+1 + (a^3-a)*h*y2_j + (b^3-b)*h*y2_j+1 with
+        // a, b from separately rounded knot times (pycbf/cpu/pycbf.c:93-103):
+        // a + b differs from 1 by up to a few 1e-5, and that is kept
+        int j = 0;
+        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+        if (!ok) return;
+        const float xlead = A.t0 + A.dt * (float)(j + 1);
+        const float xlag = A.t0 + A.dt * (float)j;
+        const float a = (xlead - x) * A.rdt;
+        const float b = (x - xlag) * A.rdt;
+        const float ca = __builtin_fmaf(a * a, a, -a);
+        const float cb = __builtin_fmaf(b * b, b, -b);
+        // entries j and j+1 of the (y, h*y2) table: one 16-byte load at 8-byte alignment
+        const float *s = A.src + ch0 * A.ch_stride + 2 * j;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) {
+                const Pair2 c = *reinterpret_cast<const Pair2 *>(s + f * fstride);
+                float r = a * c.y0;
+                r = __builtin_fmaf(b, c.y1, r);
+                r = __builtin_fmaf(ca, c.z0, r);
+                v[f] = __builtin_fmaf(cb, c.z1, r);
+            }
+    } else {
+        int j = 0;
+        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+        if (!ok) return;
+        const float t = (GK == GK_KORDER) ? ((x - A.t0) - A.dt * (float)j) * A.rdt
+                                          : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
+        const float4 *s = reinterpret_cast<const float4 *>(A.src) + ch0 * (A.ch_stride >> 2) + j;
+        const int64_t fs4 = fstride >> 2;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) {
+                const float4 c = s[f * fs4];
+                v[f] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, t, c.z), t, c.y), t, c.x);
+            }
+    }
+}
+
+}  // namespace pycbf

@@ -1,0 +1,463 @@
+// LDS-staged delay-and-sum gather for gfx950.
+//
+// Grid: (pixel tiles of 256, frame chunks, outer channel elements).  Tiles
+// are dispatched fastest, so the resident workgroups of one outer element
+// sweep the inner channel loop together and each channel's samples pass
+// through L2 about once (a channel wavefront).
+//
+// Per workgroup and channel the samples any of its 256 pixels can touch form
+// one contiguous window of the channel's table.  The window comes from the
+// tile's delay ranges (k_tile_bounds): the sample index is monotone in the
+// delay tau_tx + tau_rx and float addition is monotone, so
+//   [index(min_tx + min_rx), index(max_tx + max_rx)]
+// contains every index a contributing pixel selects.  The window of channel
+// b+1 is fetched into registers (coalesced, all frames of the chunk) while
+// channel b is evaluated from LDS, then written to the other LDS buffer; one
+// barrier per channel.  A window larger than the LDS slot falls back to
+// direct global gathers for that channel (a workgroup-uniform branch).
+//
+// Delays and weights are read from tables (geo 0) or evaluated in registers
+// from the synthetic-point geometry (geo 2 / 3), both with the float32
+// operations of the reference (pycbf/gpu/__engines__.cu:382-470).
+#include <math.h>
+#include <stdlib.h>
+
+#include "common.h"
+#include "das_dev.h"
+#include "prof.h"
+
+namespace pycbf {
+
+constexpr int LDS_BLOCK = 256;
+constexpr int LDS_BYTES = 40960;  // both staging buffers of one workgroup
+constexpr int LDS_NLD = 10;       // staged entries per thread and channel
+
+template <int GK>
+struct EntryOf {
+    // floats per table entry; interval j also reads entry j+1
+    static constexpr int ES = (GK == GK_CUBIC) ? 2 : ((GK == GK_KORDER || GK == GK_AKIMA) ? 4 : 1);
+    static constexpr int PAIR = (GK == GK_LINEAR || GK == GK_CUBIC) ? 1 : 0;
+};
+
+template <int GK, int FC>
+struct Stage {
+    static constexpr int ES = EntryOf<GK>::ES;
+    static constexpr int PAIR = EntryOf<GK>::PAIR;
+    static constexpr int TPF = LDS_BLOCK / FC;  // staging threads per frame
+    static constexpr int WIN_LDS = LDS_BYTES / (2 * FC * ES * 4);
+    static constexpr int WIN = (WIN_LDS < LDS_NLD * TPF) ? WIN_LDS : LDS_NLD * TPF;
+    static constexpr int NLD = (WIN + TPF - 1) / TPF;
+};
+
+// delay and weight of element e of the tx (or rx) axis at pixel p
+template <int GEO>
+__device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int64_t p,
+                                          const float (&pv)[3], float &tau, float &apod)
+{
+    if (GEO == 0) {
+        const int64_t o = (int64_t)e * A.np + p;
+        tau = tx ? A.ttx[o] : A.trx[o];
+        apod = tx ? A.atx[o] : A.arx[o];
+    } else if (tx) {
+        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
+    } else {
+        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// delay range per (tile, element) over the pixels that can contribute
+// ---------------------------------------------------------------------------
+template <int GEO>
+__global__ __launch_bounds__(256) void k_tile_bounds(DasArgs A)
+{
+    const int64_t tile = blockIdx.x;
+    const int e = blockIdx.y;  // [0, ntx): tx element, [ntx, ntx+nrx): rx element
+    const int64_t i = tile * 256 + threadIdx.x;
+    float lo = INFINITY, hi = -INFINITY;
+    if (i < A.np) {
+        const int64_t p = A.perm ? (int64_t)A.perm[i] : i;
+        float pv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+        const bool tx = e < A.ntx;
+        float tau = 0.0f, apod = 0.0f;
+        elem_eval<GEO>(A, tx, tx ? e : e - A.ntx, p, pv, tau, apod);
+        // with thresh > 0 a zero (or NaN) weight on either axis never contributes
+        const bool used = (tau == tau) && (!(A.thresh > 0.0f) || (apod != 0.0f && apod == apod));
+        if (used) lo = hi = tau;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    }
+    __shared__ float wl[4], wh[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wl[w] = lo;
+        wh[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lo = fminf(fminf(wl[0], wl[1]), fminf(wl[2], wl[3]));
+        hi = fmaxf(fmaxf(wh[0], wh[1]), fmaxf(wh[2], wh[3]));
+        float *b = const_cast<float *>(A.bounds) + (tile * (A.ntx + A.nrx) + e) * 2;
+        b[0] = lo;
+        b[1] = hi;
+    }
+}
+
+int launch_tile_bounds(const DasArgs &a, float *bounds, hipStream_t s)
+{
+    DasArgs b = a;
+    b.bounds = bounds;
+    dim3 grid((unsigned)das_tiles(a.np), (unsigned)(a.ntx + a.nrx));
+    ProfScope prof("tile_bounds", s);
+    if (a.geo == 3)
+        hipLaunchKernelGGL(k_tile_bounds<3>, grid, dim3(256), 0, s, b);
+    else if (a.geo == 2)
+        hipLaunchKernelGGL(k_tile_bounds<2>, grid, dim3(256), 0, s, b);
+    else
+        hipLaunchKernelGGL(k_tile_bounds<0>, grid, dim3(256), 0, s, b);
+    return check_launch("tile bounds");
+}
+
+// ---------------------------------------------------------------------------
+// window of one channel: entries [lo, lo + cnt) of its table row (cnt 0: no
+// contributing pixel selects a sample of this channel)
+// ---------------------------------------------------------------------------
+template <int GK>
+__device__ __forceinline__ void channel_window(const DasArgs &A, float xlo, float xhi, int &lo,
+                                               int &cnt)
+{
+    lo = 0;
+    cnt = 0;
+    if (!(xlo <= xhi)) return;
+    const int n = A.nsrc;
+    if (GK == GK_NEAREST) {
+        const double dlo = (double)((xlo - A.t0) / A.dt) + 0.5;
+        const double dhi = (double)((xhi - A.t0) / A.dt) + 0.5;
+        if (!(dhi > -1.0) || !(dlo < (double)n)) return;
+        const int ilo = (dlo > -1.0) ? (int)dlo : 0;
+        const int ihi = (dhi < (double)n) ? (int)dhi : n - 1;
+        lo = ilo;
+        cnt = ihi - ilo + 1;
+    } else {
+        if (xhi < A.t0 || xlo > A.tmax) return;
+        const int jlo = (xlo < A.t0) ? 0 : min((int)((xlo - A.t0) / A.dt), n - 1);
+        const int jhi = (xhi > A.tmax) ? n - 1 : min((int)((xhi - A.t0) / A.dt), n - 1);
+        lo = jlo;
+        cnt = jhi - jlo + 1 + EntryOf<GK>::PAIR;
+    }
+}
+
+template <int ES>
+struct Ent {
+    float v[ES];
+};
+
+template <int GK, int SUM, int FC, int GEO>
+__global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
+{
+    using St = Stage<GK, FC>;
+    constexpr int ES = St::ES, WIN = St::WIN, TPF = St::TPF, NLD = St::NLD;
+    __shared__ __align__(16) float lds[2 * FC * WIN * ES];
+
+    uint32_t bx, by, bz;
+    block_coords(A.xcd_remap, bx, by, bz);
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)bx * LDS_BLOCK + tid;
+    const bool live = i < A.np;
+    const int64_t p = live ? (A.perm ? (int64_t)A.perm[i] : i) : 0;
+    const int f0 = by * FC;
+    const int nf = min(FC, A.nfr - f0);
+    const int64_t np = A.np;
+
+    const bool OUTER_TX = (SUM == PYCBF_SUM_RX_ONLY) || (SUM != PYCBF_SUM_TX_ONLY && A.outer_tx);
+    const int na = OUTER_TX ? A.ntx : A.nrx;
+    const int nb = OUTER_TX ? A.nrx : A.ntx;
+    const int a0 = bz * A.a_split;
+    const int a1 = min(na, a0 + A.a_split);
+    const int64_t nchan = (int64_t)A.ntx * A.nrx;
+    const int64_t fstride = nchan * A.ch_stride;
+    const float *bnd = A.bounds + (int64_t)bx * (A.ntx + A.nrx) * 2;
+    const int boff_a = OUTER_TX ? 0 : A.ntx;  // bounds index offset of the outer axis
+    const int boff_b = OUTER_TX ? A.ntx : 0;
+
+    float pv[3] = {0.0f, 0.0f, 0.0f};
+    if (GEO > 0 && live) {
+#pragma unroll
+        for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+    }
+    const int sfr = tid / TPF;  // staging role: frame and first entry
+    const int se0 = tid % TPF;
+
+    float acc[FC];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) acc[f] = 0.0f;
+
+    for (int a = a0; a < a1; ++a) {
+        float ta = 0.0f, wa = 0.0f;
+        if (live) elem_eval<GEO>(A, OUTER_TX, a, p, pv, ta, wa);
+        const float amin = bnd[2 * (boff_a + a)], amax = bnd[2 * (boff_a + a) + 1];
+
+        Ent<ES> reg[NLD];
+        auto window = [&](int b, int &lo, int &cnt) {
+            const float bmin = bnd[2 * (boff_b + b)], bmax = bnd[2 * (boff_b + b) + 1];
+            channel_window<GK>(A, amin + bmin, amax + bmax, lo, cnt);
+        };
+        auto fetch = [&](int b, int lo, int cnt) {
+            if (cnt == 0 || cnt > WIN || sfr >= nf) return;
+            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
+            const float *base =
+                A.src + ((int64_t)(f0 + sfr) * nchan + (int64_t)tx * A.nrx + rx) * A.ch_stride;
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int e = se0 + k * TPF;
+                if (e < cnt) {
+                    const int g = lo + e;
+                    if (g < A.nent) {
+                        reg[k] = *reinterpret_cast<const Ent<ES> *>(base + (int64_t)g * ES);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < ES; ++c) reg[k].v[c] = 0.0f;
+                    }
+                }
+            }
+        };
+        auto store = [&](int buf, int cnt) {
+            if (cnt == 0 || cnt > WIN || sfr >= nf) return;
+            float *dst = lds + ((buf * FC + sfr) * WIN) * ES;
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int e = se0 + k * TPF;
+                if (e < cnt) *reinterpret_cast<Ent<ES> *>(dst + e * ES) = reg[k];
+            }
+        };
+
+        int lo_c, cnt_c;
+        window(0, lo_c, cnt_c);
+        fetch(0, lo_c, cnt_c);
+        store(0, cnt_c);
+        __syncthreads();
+        for (int b = 0; b < nb; ++b) {
+            const int buf = b & 1;
+            int lo_n = 0, cnt_n = 0;
+            if (b + 1 < nb) {
+                window(b + 1, lo_n, cnt_n);
+                fetch(b + 1, lo_n, cnt_n);
+            }
+            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
+            if (cnt_c > 0) {
+                float tb = 0.0f, wb = 0.0f;
+                if (live) elem_eval<GEO>(A, !OUTER_TX, b, p, pv, tb, wb);
+                const float w = OUTER_TX ? wa * wb : wb * wa;  // apodtx * apodrx
+                const float x = OUTER_TX ? ta + tb : tb + ta;  // tautx + taurx
+                const bool use = live && cnt_c > 0 && fabsf(w) >= A.thresh;
+                float v[FC];
+                bool ok = false;
+                if (use && cnt_c > WIN) {
+                    // window too large for the LDS slot: direct global gather
+                    gather<GK, FC>(A, (int64_t)f0 * nchan + (int64_t)tx * A.nrx + rx, fstride, nf, x,
+                                   ok, v);
+                } else if (use) {
+                    const float *L = lds + (buf * FC * WIN) * ES;
+                    if (GK == GK_NEAREST) {
+                        int it = 0;
+                        ok = nearest_index(x, A.t0, A.dt, A.nsrc, it);
+                        if (ok) {
+                            const int e = it - lo_c;
+    #pragma unroll
+                            for (int f = 0; f < FC; ++f)
+                                if (f < nf) v[f] = L[f * WIN + e];
+                        }
+                    } else if (GK == GK_LINEAR) {
+                        int j = 0;
+                        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+                        if (ok) {
+                            const float xlead = A.t0 + A.dt * (float)(j + 1);
+                            const float xlag = A.t0 + A.dt * (float)j;
+                            const float ca = (xlead - x) / A.dt;
+                            const float cb = (x - xlag) / A.dt;
+                            const int e = j - lo_c;
+    #pragma unroll
+                            for (int f = 0; f < FC; ++f)
+                                if (f < nf) v[f] = ca * L[f * WIN + e] + cb * L[f * WIN + e + 1];
+                        }
+                    } else if (GK == GK_CUBIC) {
+                        int j = 0;
+                        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+                        if (ok) {
+                            const float xlead = A.t0 + A.dt * (float)(j + 1);
+                            const float xlag = A.t0 + A.dt * (float)j;
+                            float ka = (xlead - x) * A.rdt;
+                            float kb = (x - xlag) * A.rdt;
+                            float kc = __builtin_fmaf(ka * ka, ka, -ka);
+                            float kd = __builtin_fmaf(kb * kb, kb, -kb);
+                            const float *E = L + (j - lo_c) * 2;
+                            if (SUM == PYCBF_SUM_NONE) {
+    #pragma unroll
+                                for (int f = 0; f < FC; ++f)
+                                    if (f < nf) {
+                                        const float *Ef = E + f * WIN * 2;
+                                        float r = ka * Ef[0];
+                                        r = __builtin_fmaf(kb, Ef[2], r);
+                                        r = __builtin_fmaf(kc, Ef[1], r);
+                                        v[f] = __builtin_fmaf(kd, Ef[3], r);
+                                    }
+                            } else {
+                                // fold the weight into the four coefficients: four
+                                // fused multiply-adds per frame straight into acc
+                                ka *= w;
+                                kb *= w;
+                                kc *= w;
+                                kd *= w;
+    #pragma unroll
+                                for (int f = 0; f < FC; ++f)
+                                    if (f < nf) {
+                                        const float *Ef = E + f * WIN * 2;
+                                        float r = __builtin_fmaf(ka, Ef[0], acc[f]);
+                                        r = __builtin_fmaf(kb, Ef[2], r);
+                                        r = __builtin_fmaf(kc, Ef[1], r);
+                                        acc[f] = __builtin_fmaf(kd, Ef[3], r);
+                                    }
+                                ok = false;  // already accumulated
+                            }
+                        }
+                    } else {
+                        int j = 0;
+                        ok = knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+                        if (ok) {
+                            const float t = (GK == GK_KORDER) ? ((x - A.t0) - A.dt * (float)j) * A.rdt
+                                                              : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
+                            const float *E = L + (j - lo_c) * 4;
+    #pragma unroll
+                            for (int f = 0; f < FC; ++f)
+                                if (f < nf) {
+                                    const float4 c = *reinterpret_cast<const float4 *>(E + f * WIN * 4);
+                                    v[f] = __builtin_fmaf(
+                                        __builtin_fmaf(__builtin_fmaf(c.w, t, c.z), t, c.y), t, c.x);
+                                }
+                        }
+                    }
+                }
+                if (SUM == PYCBF_SUM_NONE) {
+                    float *o = A.out + ((((int64_t)f0 * A.ntx + tx) * A.nrx + rx) * np + p);
+                    if (live) {
+    #pragma unroll
+                        for (int f = 0; f < FC; ++f)
+                            if (f < nf) o[(int64_t)f * nchan * np] = ok ? 0.0f + w * v[f] : 0.0f;
+                    }
+                } else if (ok) {
+    #pragma unroll
+                    for (int f = 0; f < FC; ++f)
+                        if (f < nf) acc[f] = __builtin_fmaf(w, v[f], acc[f]);
+                }
+            } else if (SUM == PYCBF_SUM_NONE && live) {
+                // no pixel of this tile takes a sample of this channel
+                float *o = A.out + ((((int64_t)f0 * A.ntx + tx) * A.nrx + rx) * np + p);
+#pragma unroll
+                for (int f = 0; f < FC; ++f)
+                    if (f < nf) o[(int64_t)f * nchan * np] = 0.0f;
+            }
+            if (b + 1 < nb) store(buf ^ 1, cnt_n);
+            // workgroup-uniform: a barrier is needed only when this channel was
+            // read from LDS (its buffer is refilled next) or the next was staged
+            const bool staged_c = cnt_c > 0 && cnt_c <= WIN;
+            const bool staged_n = cnt_n > 0 && cnt_n <= WIN;
+            if (staged_c || staged_n) __syncthreads();
+            lo_c = lo_n;
+            cnt_c = cnt_n;
+        }
+        if (SUM == PYCBF_SUM_TX_ONLY || SUM == PYCBF_SUM_RX_ONLY) {
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+                if (live && f < nf) A.out[((int64_t)(f0 + f) * na + a) * np + p] = acc[f];
+                acc[f] = 0.0f;
+            }
+        }
+    }
+    if (SUM == PYCBF_SUM_TX_AND_RX && live) {
+        float *o = A.out + (int64_t)bz * A.nfr * np;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) o[(int64_t)(f0 + f) * np + p] = acc[f];
+    }
+}
+
+template <int GK, int SUM, int FC>
+static void lds_geo(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    if (geo == 3)
+        hipLaunchKernelGGL((k_das_lds<GK, SUM, FC, 3>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else if (geo == 2)
+        hipLaunchKernelGGL((k_das_lds<GK, SUM, FC, 2>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_das_lds<GK, SUM, FC, 0>), grid, dim3(LDS_BLOCK), 0, s, a);
+}
+
+template <int GK, int SUM>
+static void lds_fc(int fc, const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    if (fc >= 16)
+        lds_geo<GK, SUM, 16>(a.geo, a, grid, s);
+    else if (fc >= 4)
+        lds_geo<GK, SUM, 4>(a.geo, a, grid, s);
+    else
+        lds_geo<GK, SUM, 1>(a.geo, a, grid, s);
+}
+
+template <int GK>
+static void lds_sum(int fc, const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    switch (a.sumflag) {
+    case PYCBF_SUM_NONE: lds_fc<GK, PYCBF_SUM_NONE>(fc, a, grid, s); break;
+    case PYCBF_SUM_TX_ONLY: lds_fc<GK, PYCBF_SUM_TX_ONLY>(fc, a, grid, s); break;
+    case PYCBF_SUM_RX_ONLY: lds_fc<GK, PYCBF_SUM_RX_ONLY>(fc, a, grid, s); break;
+    default: lds_fc<GK, PYCBF_SUM_TX_AND_RX>(fc, a, grid, s); break;
+    }
+}
+
+// frames per workgroup for the staged gather: 16, 4 or 1
+int das_lds_frames_per_pass(int64_t nfr)
+{
+    int cap = 16;
+    if (const char *e = getenv("PYCBF_DAS_FC")) cap = atoi(e);
+    if (nfr > 8 && cap >= 16) return 16;
+    if (nfr > 1 && cap >= 4) return 4;
+    return 1;
+}
+
+int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStream_t s)
+{
+    DasArgs a = a0;
+    const int fc = das_lds_frames_per_pass(a.nfr);
+    a.xcd_remap = 0;
+    if (const char *e = getenv("PYCBF_XCD_REMAP")) a.xcd_remap = atoi(e);
+    a.outer_tx = das_outer_is_tx(a.sumflag, a.ntx, a.nrx) ? 1 : 0;
+    const int na = a.outer_tx ? a.ntx : a.nrx;
+    if (nsplit < 1) nsplit = 1;
+    if (nsplit > na) nsplit = na;
+    a.a_split = (na + nsplit - 1) / nsplit;
+    nsplit = (na + a.a_split - 1) / a.a_split;
+    const bool reduce = (a.sumflag == PYCBF_SUM_TX_AND_RX) && nsplit > 1;
+    float *final_out = a.out;
+    if (reduce) a.out = partial;
+    dim3 grid((unsigned)das_tiles(a.np), (unsigned)((a.nfr + fc - 1) / fc), (unsigned)nsplit);
+    {
+        ProfScope prof("das_gather", s);
+        switch (gk) {
+        case GK_NEAREST: lds_sum<GK_NEAREST>(fc, a, grid, s); break;
+        case GK_LINEAR: lds_sum<GK_LINEAR>(fc, a, grid, s); break;
+        case GK_CUBIC: lds_sum<GK_CUBIC>(fc, a, grid, s); break;
+        case GK_KORDER: lds_sum<GK_KORDER>(fc, a, grid, s); break;
+        default: lds_sum<GK_AKIMA>(fc, a, grid, s); break;
+        }
+    }
+    int rc = check_launch("das gather (lds)");
+    if (rc || !reduce) return rc;
+    return launch_reduce_split(partial, nsplit, (int64_t)a.nfr * a.np, final_out, s);
+}
+
+}  // namespace pycbf

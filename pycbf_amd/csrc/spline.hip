@@ -24,61 +24,105 @@ namespace pycbf {
 // y2f[i] = (0.5 - 1) / (0.5*y2f[i-1] + 2), reach an exact float32 fixed point
 // at i = 7; the kernels keep the first 32 and use y2f[31] for every later i.
 
-// 16 floats of a lane-private row, read or written as four 16-byte accesses
-// (rows need only 4-byte alignment; gfx950 serves unaligned dwordx4 accesses).
+// 16-byte unit of a row; rows need only 4-byte alignment (gfx950 serves
+// unaligned dwordx4 accesses).
 struct __attribute__((aligned(4))) F4u {
     float v[4];
 };
 
-template <int N>
-__device__ __forceinline__ void row_load(const float *row, int s, int nt, float (&d)[N])
+// LDS staging area of the spline sweeps: 64 rows (channels) x 72 floats.
+// Forward: columns [0,32) hold the y tile, [36,68) the u tile.  Backward:
+// y tile, u tile, then [0,64) the interleaved (y, h*y2) output entries.
+constexpr int SPL_PITCH = 72;
+constexpr int SPL_UOFF = 36;
+typedef float SplRow[SPL_PITCH];
+
+// A 64-row x 32-column tile moves between global memory and registers in
+// whole cache lines: lane l covers row 8q + l/8, columns 4*(l%8)..4*(l%8)+3,
+// so every instruction reads or writes eight complete 128-byte lines.
+struct TileRegs {
+    F4u q[8];
+};
+
+__device__ __forceinline__ void tile_fetch(const float *base, int64_t ld, int nrow, int s, int ncol,
+                                           TileRegs &t)
 {
-    if (s + N <= nt) {
+    const int lane = threadIdx.x;
+    const int c = 4 * (lane & 7);
 #pragma unroll
-        for (int q = 0; q < N / 4; ++q) {
-            const F4u w = *reinterpret_cast<const F4u *>(row + s + 4 * q);
+    for (int q = 0; q < 8; ++q) {
+        const int r = 8 * q + (lane >> 3);
+        F4u w{{0.0f, 0.0f, 0.0f, 0.0f}};
+        if (r < nrow) {
+            const float *src = base + (int64_t)r * ld + s + c;
+            if (s + c + 4 <= ncol) {
+                w = *reinterpret_cast<const F4u *>(src);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) d[4 * q + e] = w.v[e];
+                for (int e = 0; e < 4; ++e)
+                    if (s + c + e < ncol) w.v[e] = src[e];
+            }
         }
-    } else {
-#pragma unroll
-        for (int c = 0; c < N; ++c) d[c] = (s + c < nt) ? row[s + c] : 0.0f;
+        t.q[q] = w;
     }
 }
 
-template <int N>
-__device__ __forceinline__ void row_store(float *row, int s, int nt, const float (&d)[N])
+__device__ __forceinline__ void tile_to_lds(const TileRegs &t, SplRow *S, int coff)
 {
-    if (s + N <= nt) {
+    const int lane = threadIdx.x;
+    const int c = coff + 4 * (lane & 7);
 #pragma unroll
-        for (int q = 0; q < N / 4; ++q) {
-            F4u w;
+    for (int q = 0; q < 8; ++q)
+        *reinterpret_cast<float4 *>(&S[8 * q + (lane >> 3)][c]) =
+            make_float4(t.q[q].v[0], t.q[q].v[1], t.q[q].v[2], t.q[q].v[3]);
+}
+
+__device__ __forceinline__ void lds_to_global(const SplRow *S, int coff, float *base, int64_t ld,
+                                              int nrow, int s, int ncol)
+{
+    const int lane = threadIdx.x;
+    const int c = 4 * (lane & 7);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) w.v[e] = d[4 * q + e];
-            *reinterpret_cast<F4u *>(row + s + 4 * q) = w;
+    for (int q = 0; q < 8; ++q) {
+        const int r = 8 * q + (lane >> 3);
+        if (r >= nrow) continue;
+        const float4 v = *reinterpret_cast<const float4 *>(&S[r][coff + c]);
+        float *dst = base + (int64_t)r * ld + s + c;
+        if (s + c + 4 <= ncol) {
+            F4u w{{v.x, v.y, v.z, v.w}};
+            *reinterpret_cast<F4u *>(dst) = w;
+        } else {
+            const float e4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (s + c + e < ncol) dst[e] = e4[e];
         }
-    } else {
-#pragma unroll
-        for (int c = 0; c < N; ++c)
-            if (s + c < nt) row[s + c] = d[c];
     }
 }
 
-// One lane per channel.  Each lane streams its own row in 32-sample chunks
-// with the next chunk prefetched into registers.  Per chunk the data-parallel
-// part of the forward sweep, g_i = 6*((y[i+1]-y[i])/dx - (y[i]-y[i-1])/dx)/(2*dx),
-// is computed first (32 independent lanes of work), so only
-// u_i = (g_i - 0.5*u_{i-1}) / p_i stays on the serial dependency chain.  The
-// operations and their order are exactly those of tie_knots1D_fixed
-// (pycbf/cpu/pycbf.c:126-141), so y2 is bit-identical.
+// One wave per 64 channels, one lane per channel for the recurrences.  Tiles
+// of 32 samples move through LDS (coalesced global traffic, transposed to one
+// row per lane) with the next tile prefetched into registers.  Per tile the
+// data-parallel part of the forward sweep, g_i = 6*((y[i+1]-y[i])/dx -
+// (y[i]-y[i-1])/dx)/(2*dx), is evaluated first (32 independent evaluations
+// per lane), so only u_i = (g_i - 0.5*u_{i-1}) / p_i remains on the serial
+// chain.  Operations and their order are those of tie_knots1D_fixed
+// (pycbf/cpu/pycbf.c:126-141): y2 is bit-identical.
 template <int MODE>  // 0: (y, dx^2/6*y2) gather table, 1: y2
 __global__ __launch_bounds__(64) void k_spline(const float *__restrict__ rf, int64_t nch, int nt,
                                                float dx, float *__restrict__ uscr,
                                                void *__restrict__ outp)
 {
     constexpr int CH = SPL_CHUNK;
-    const int64_t ch = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (ch >= nch) return;
+    static_assert(CH == 32, "tile helpers assume 32-sample tiles");
+    __shared__ __align__(16) SplRow S[64];
+
+    const int lane = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int nrow = (int)((nch - c0) < 64 ? (nch - c0) : 64);
+    const bool live = lane < nrow;
+    const float *ybase = rf + c0 * (int64_t)nt;
+    float *ubase = uscr + c0 * (int64_t)nt;
 
     float y2f[32];
     {
@@ -91,32 +135,36 @@ __global__ __launch_bounds__(64) void k_spline(const float *__restrict__ rf, int
             y2f[i] = v;
         }
     }
-    // p_i = 0.5*y2f[i-1] + 2 for i >= 32 (fixed point)
-    const float p_inf = 0.5f * y2f[31] + 2.0f;
-
-    const float *row = rf + ch * (int64_t)nt;
-    float *urow = uscr + ch * (int64_t)nt;
+    const float p_inf = 0.5f * y2f[31] + 2.0f;  // p_i for every i >= 32
     const int nchunk = (nt + CH - 1) / CH;
 
     // ---- forward sweep ---------------------------------------------------
-    float ycur[CH], ynext[CH];
-    row_load<CH>(row, 0, nt, ycur);
-    float y_prev = 0.0f, u_prev = 0.0f;
+    TileRegs pf;
+    tile_fetch(ybase, nt, nrow, 0, nt, pf);
+    tile_to_lds(pf, S, 0);
+    if (nchunk > 1) tile_fetch(ybase, nt, nrow, CH, nt, pf);
+    __syncthreads();
+    // slope (y[i] - y[i-1])/dx of the previous step: the reference evaluates
+    // both slopes at every step, and this one equals the previous step's
+    // forward slope (same operands, same correctly rounded division)
+    float d_prev = 0.0f, u_prev = 0.0f;
     for (int cb = 0; cb < nchunk; ++cb) {
         const int s = cb * CH;
-        if (cb + 1 < nchunk) row_load<CH>(row, s + CH, nt, ynext);
-        const float y_after = (cb + 1 < nchunk) ? ynext[0] : 0.0f;
+        float y[CH + 1];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) y[c] = S[lane][c];
+        y[CH] = (live && s + CH < nt) ? ybase[(int64_t)lane * nt + s + CH] : 0.0f;
         float g[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            const float yp = (c == 0) ? y_prev : ycur[c - 1];
-            const float yn = (c == CH - 1) ? y_after : ycur[c + 1];
-            const float jump = (yn - ycur[c]) / dx - (ycur[c] - yp) / dx;
+            const float d = (y[c + 1] - y[c]) / dx;
+            const float jump = d - d_prev;
             g[c] = 6.0f * jump / (2.0f * dx);
+            d_prev = d;
         }
         float u[CH];
         if (s >= 32 && s + CH <= nt - 1) {
-            // interior chunk: p is the fixed point, every i in [1, nt-2]
+            // interior tile: p is the fixed point and every i is in [1, nt-2]
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
                 u_prev = (g[c] - 0.5f * u_prev) / p_inf;
@@ -135,62 +183,84 @@ __global__ __launch_bounds__(64) void k_spline(const float *__restrict__ rf, int
                 u[c] = ui;
             }
         }
-        row_store<CH>(urow, s, nt, u);
-        y_prev = ycur[CH - 1];
 #pragma unroll
-        for (int c = 0; c < CH; ++c) ycur[c] = ynext[c];
+        for (int c = 0; c < CH; c += 4)
+            *reinterpret_cast<float4 *>(&S[lane][SPL_UOFF + c]) =
+                make_float4(u[c], u[c + 1], u[c + 2], u[c + 3]);
+        __syncthreads();  // u tile complete; every lane is done with the y tile
+        lds_to_global(S, SPL_UOFF, ubase, nt, nrow, s, nt);
+        if (cb + 1 < nchunk) {
+            tile_to_lds(pf, S, 0);
+            if (cb + 2 < nchunk) tile_fetch(ybase, nt, nrow, s + 2 * CH, nt, pf);
+        }
+        __syncthreads();  // next y tile complete; u tile read out
     }
+    // u rows are read back below by the lanes that wrote them
+    __threadfence();
 
     // ---- back substitution: y2[k] = y2f[k]*y2[k+1] + u[k] ------------------
     const float h = (1.0f / 6.0f) * (dx * dx);
     float y2_next = 0.0f;  // y2[k+1]; zero at the natural end and beyond
-    float ucur[CH], unext[CH];
     const int slast = (nchunk - 1) * CH;
-    row_load<CH>(row, slast, nt, ycur);
-    row_load<CH>(urow, slast, nt, ucur);
-    float2 *tab = static_cast<float2 *>(outp) + ch * (int64_t)(nt + 1);
-    float *y2row = static_cast<float *>(outp) + ch * (int64_t)nt;
-    if (MODE == 0) tab[nt] = make_float2(0.0f, 0.0f);
+    TileRegs py, pu;
+    if (MODE == 0) tile_fetch(ybase, nt, nrow, slast, nt, py);
+    tile_fetch(ubase, nt, nrow, slast, nt, pu);
+    if (MODE == 0 && live) {
+        float2 *tab = static_cast<float2 *>(outp) + (c0 + lane) * (int64_t)(nt + 1);
+        tab[nt] = make_float2(0.0f, 0.0f);
+    }
     for (int cb = nchunk - 1; cb >= 0; --cb) {
         const int s = cb * CH;
+        if (MODE == 0) tile_to_lds(py, S, 0);
+        tile_to_lds(pu, S, SPL_UOFF);
+        __syncthreads();
         if (cb > 0) {
-            row_load<CH>(row, s - CH, nt, ynext);
-            row_load<CH>(urow, s - CH, nt, unext);
+            if (MODE == 0) tile_fetch(ybase, nt, nrow, s - CH, nt, py);
+            tile_fetch(ubase, nt, nrow, s - CH, nt, pu);
         }
-        float out[CH];
+        float yr[CH], out[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) yr[c] = (MODE == 0) ? S[lane][c] : 0.0f;
 #pragma unroll
         for (int c = CH - 1; c >= 0; --c) {
             const int k = s + c;
             float y2k = 0.0f;
-            if (k < nt - 1) y2k = y2f[k < 31 ? k : 31] * y2_next + ucur[c];
+            if (k < nt - 1) y2k = y2f[k < 31 ? k : 31] * y2_next + S[lane][SPL_UOFF + c];
             if (k < nt) y2_next = y2k;
             out[c] = (MODE == 0) ? h * y2k : y2k;
         }
+        __syncthreads();  // every lane has read its y and u rows
         if (MODE == 0) {
-            // entries (y_k, h*y2_k), two per 16-byte store
-            if (s + CH <= nt) {
 #pragma unroll
-                for (int q = 0; q < CH / 2; ++q) {
-                    F4u w;
-                    w.v[0] = ycur[2 * q];
-                    w.v[1] = out[2 * q];
-                    w.v[2] = ycur[2 * q + 1];
-                    w.v[3] = out[2 * q + 1];
-                    *reinterpret_cast<F4u *>(tab + s + 2 * q) = w;
+            for (int c = 0; c < CH; c += 2)
+                *reinterpret_cast<float4 *>(&S[lane][2 * c]) =
+                    make_float4(yr[c], out[c], yr[c + 1], out[c + 1]);
+            __syncthreads();
+            // row r: 32 entries = 256 contiguous bytes, 16 lanes per row
+            float2 *tab = static_cast<float2 *>(outp) + c0 * (int64_t)(nt + 1);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int r = 4 * q + (lane >> 4), e = 2 * (lane & 15);
+                if (r >= nrow) continue;
+                float2 *dst = tab + (int64_t)r * (nt + 1) + s + e;
+                const float4 v = *reinterpret_cast<const float4 *>(&S[r][2 * e]);
+                if (s + e + 2 <= nt) {
+                    F4u w{{v.x, v.y, v.z, v.w}};
+                    *reinterpret_cast<F4u *>(dst) = w;
+                } else if (s + e < nt) {
+                    dst[0] = make_float2(v.x, v.y);
                 }
-            } else {
-#pragma unroll
-                for (int c = 0; c < CH; ++c)
-                    if (s + c < nt) tab[s + c] = make_float2(ycur[c], out[c]);
             }
         } else {
-            row_store<CH>(y2row, s, nt, out);
-        }
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            ycur[c] = ynext[c];
-            ucur[c] = unext[c];
+            for (int c = 0; c < CH; c += 4)
+                *reinterpret_cast<float4 *>(&S[lane][c]) =
+                    make_float4(out[c], out[c + 1], out[c + 2], out[c + 3]);
+            __syncthreads();
+            float *y2 = static_cast<float *>(outp) + c0 * (int64_t)nt;
+            lds_to_global(S, 0, y2, nt, nrow, s, nt);
         }
+        __syncthreads();  // staging area free for the next tile
     }
 }
 
