@@ -58,6 +58,27 @@ def make_rf(steers, xele, nframes, device, seed=1234):
     return rf
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                               "das_gather_synthetic_cubic_75x128x512x512_nt4096_16fr.md")
+
+
+def measured_traffic(frames):
+    """HBM-side bytes per gather launch from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), when it
+    was measured for this frame count; None otherwise."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "frames"):
+                k, v = line.split(":", 1)
+                vals[k] = float(v)
+    if vals.get("frames") != frames or "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"]
+
+
 def cpu_baseline(rf0, params, stride, threads):
     """Reference C cubic DAS (oracle/_ref, pycbf/cpu/pycbf.c) -- or the oracle
     restatement when the reference build is absent -- on frame 0 over a strided
@@ -176,13 +197,7 @@ def main():
     achieved = algo_bytes / (gather_ms * 1e-3) / 1e9
     log(f"gather {gather_ms:.3f} ms/launch, spline {s_ms / max(s_n, 1):.3f} ms, "
         f"tables {t_ms / max(t_n, 1):.3f} ms")
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "das_gather_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tr = json.load(fh)
-        if tr.get("frames") == args.frames:
-            traffic = tr.get("hbm_bytes_per_launch")
+    traffic = measured_traffic(args.frames)
 
     cpu = None
     if not args.no_cpu:
