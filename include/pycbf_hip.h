@@ -20,6 +20,7 @@
  *   pycbf/gpu/__engines__.cu:360-374     struct xInfo / RFInfo (same layout)
  *   pycbf/gpu/__engines__.cu:506-864     das_bmode_{synthetic,tabbed}_{korder_cubic,multi_interp}
  *   pycbf/gpu/__advanced_engines__.cu:17-61  calc_dmas
+ *   examples/advanced_beamforming.ipynb:65-72,205-223  SLSC (calc_lagn_coherence + pixel loop)
  *   pycbf/cpu/pycbf.h:23-54              tie_knots1D_fixed / beamform_cubic / beamform_nearest
  */
 #ifndef PYCBF_HIP_H
@@ -53,6 +54,12 @@ extern "C" {
 #define PYCBF_DMAS_SSR 0
 #define PYCBF_DMAS_POWER 1
 #define PYCBF_DMAS_PSSR 2
+
+/* SLSC arithmetic: FP32 on the VALU, or FP16 / BF16 normalised channels on MFMA
+ * with FP32 accumulation */
+#define PYCBF_SLSC_FP32 0
+#define PYCBF_SLSC_MFMA_FP16 1
+#define PYCBF_SLSC_MFMA_BF16 2
 
 /* floats per transmit / receive element in the synthetic geometry packs */
 #define PYCBF_TXPACK_FLOATS 16
@@ -163,10 +170,32 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                int nlag, int dmastype, int sumlags, float *out, void *stream);
 
+/* Short-lag spatial coherence.  Replaces calc_lagn_coherence and the pixel loop
+ * of examples/advanced_beamforming.ipynb:65-72,205-223 (the reference has no
+ * GPU SLSC).
+ *   im    (nfr, nrx, nx, nz) per-receive-channel images (DAS with sumtype
+ *         tx_only over an x-major grid of nx * nz pixels)
+ *   apod  (nrx, nx, nz) receive apodisation, or NULL (every element selected)
+ *   out   (nfr, ceil(nx/dsfx), ceil(nz/dsfz)); output (ixo, izo) is pixel
+ *         (ixo*dsfx, izo*dsfz) with axial window [iz+kofs, iz+kofs+kwin-1]
+ *         clipped to the image.  The notebook's window iz-nkern..iz+nkern is
+ *         kwin = 2*nkern+1, kofs = -nkern.
+ * An element is used when apod is non-zero over the whole window; with fewer
+ * than max(lags)+1 such elements the output is 0.  Per lag, the pair-normalised
+ * window correlation is averaged over pairs of used elements that lag apart
+ * (pairs of an element with standard deviation < 1e-10 are dropped; a lag with
+ * no pair left is NaN); the output is the mean over lags.
+ *   lags  HOST array of nlag (1..64) lag values in 1..63
+ *   limits: 1 <= kwin <= 32, -(kwin-1) <= kofs <= 0, nrx <= 256, dsfx, dsfz >= 1
+ *   precision  PYCBF_SLSC_FP32 / _MFMA_FP16 / _MFMA_BF16 */
+int pycbf_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
+               const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+               int nlag, int precision, float *out, void *stream);
+
 /* Per-kernel timing: when enabled, instrumented launches record HIP events on
  * their stream; pycbf_profile_read synchronises them and returns the summed
  * duration (ms) and launch count of kernel `kernel` ("das_gather",
- * "spline_coefs", "synthetic_tables", ...).  No reference counterpart. */
+ * "spline_coefs", "synthetic_tables", "slsc_fp32", "slsc_mfma_fp16", ...).  No reference counterpart. */
 void pycbf_profile_enable(int on);
 void pycbf_profile_reset(void);
 int pycbf_profile_read(const char *kernel, double *total_ms, int64_t *launches);

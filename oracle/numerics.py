@@ -14,6 +14,12 @@
   mean-centre each channel over the kernel window, population std (std <
   1e-10 -> NaN), pair-normalised cross correlation averaged over valid pairs
   of *selected* elements (nanmean), then averaged over the lags.
+* ``slsc_image`` restates the notebook's pixel loop itself
+  (examples/advanced_beamforming.ipynb:205-223) over a per-receive-channel
+  image (nrx, nx, nz): output pixel (ix, iz) = (ixo*dsfx, izo*dsfz), axial
+  window iz+kofs .. iz+kofs+kwin-1 clipped to the image (the notebook's
+  iz-nkern .. iz+nkern is kwin = 2*nkern+1, kofs = -nkern), elements selected
+  where the apodisation is non-zero over the whole window.
 """
 from __future__ import annotations
 
@@ -100,4 +106,32 @@ def slsc(X, mask, lags):
         for m in lags:
             acc += lag_coherence(sub, m) / len(lags)
         out[p] = acc
+    return out
+
+
+def slsc_image(im, apod, lags, kwin, kofs, dsfx=1, dsfz=1):
+    """SLSC image of one frame, FP64 (pure-Python pixel loop: small cases).
+
+    im: (nrx, nx, nz); apod: (nrx, nx, nz) or None; returns
+    (ceil(nx/dsfx), ceil(nz/dsfz)) float64."""
+    im = np.asarray(im, np.float64)
+    nrx, nx, nz = im.shape
+    lags = [int(m) for m in np.asarray(lags).ravel()]
+    need = max(lags) + 1
+    out = np.zeros(((nx + dsfx - 1) // dsfx, (nz + dsfz - 1) // dsfz))
+    for ix in range(0, nx, dsfx):
+        for iz in range(0, nz, dsfz):
+            z0 = max(0, iz + kofs)
+            z1 = min(nz, iz + kofs + kwin)
+            if apod is None:
+                sel = np.ones(nrx, bool)
+            else:
+                sel = np.all(np.asarray(apod)[:, ix, z0:z1] != 0, axis=-1)
+            if np.sum(sel) < need:
+                continue
+            channels = im[sel, ix, z0:z1]
+            acc = 0.0
+            for m in lags:
+                acc += lag_coherence(channels.T, m) / len(lags)
+            out[ix // dsfx, iz // dsfz] = acc
     return out

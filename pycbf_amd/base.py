@@ -215,3 +215,54 @@ class DMASBeamformer:
             raise BeamformerException("lags must be integers")
         self.lags = lags.astype(np.int32)
         self.nlags = int(lags.size)
+
+
+@dataclass(kw_only=True)
+class SLSCBeamformer:
+    """Short-lag spatial coherence fields (no class in the reference; the
+    algorithm is examples/advanced_beamforming.ipynb:65-72,205-223).
+
+    The base DAS sums transmits only (per-receive-channel images); ``gridshape``
+    = (nx, nz) gives the x-major layout of the Np pixels so the axial window
+    can be taken along z.  Output pixel (ixo, izo) is pixel (ixo*dsfx,
+    izo*dsfz); its window is z + kofs .. z + kofs + kwin - 1 clipped to the
+    grid (the notebook's z - nkern .. z + nkern is kwin = 2*nkern+1,
+    kofs = -nkern, the default when kofs is None).  ``precision`` 'fp32'
+    computes the lag sums on the vector units, 'fp16' / 'bf16' on MFMA from
+    normalised channels in that format (FP32 accumulation)."""
+    lags: np.ndarray = field(init=True)
+    gridshape: tuple = field(init=True)
+    kwin: int = 8
+    kofs: int | None = None
+    dsfx: int = 1
+    dsfz: int = 1
+    precision: Literal["fp32", "fp16", "bf16"] = "fp16"
+    nlags: int = field(init=False)
+
+    def __post_init__(self):
+        self.sumtype = "tx_only"
+        lags = np.asarray(self.lags).ravel()
+        if lags.size == 0 or lags.size > 64:
+            raise BeamformerException("lags must hold 1 to 64 values")
+        if not np.all(np.equal(np.mod(lags, 1), 0)) or lags.min() < 1 or lags.max() > 63:
+            raise BeamformerException("lags must be integers in 1..63")
+        self.lags = lags.astype(np.int32)
+        self.nlags = int(lags.size)
+        if len(tuple(self.gridshape)) != 2:
+            raise BeamformerException("gridshape must be (nx, nz)")
+        self.gridshape = tuple(int(g) for g in self.gridshape)
+        self.kwin = int(self.kwin)
+        if not 1 <= self.kwin <= 32:
+            raise BeamformerException("kwin must be in 1..32")
+        self.kofs = -(self.kwin // 2) if self.kofs is None else int(self.kofs)
+        if not -(self.kwin - 1) <= self.kofs <= 0:
+            raise BeamformerException("kofs must be in -(kwin-1)..0 so the window holds its pixel")
+        if int(self.dsfx) < 1 or int(self.dsfz) < 1:
+            raise BeamformerException("dsfx and dsfz must be >= 1")
+        self.dsfx, self.dsfz = int(self.dsfx), int(self.dsfz)
+        if self.precision not in ("fp32", "fp16", "bf16"):
+            raise BeamformerException("precision must be 'fp32', 'fp16' or 'bf16'")
+
+    def _slsc_outshape(self):
+        nx, nz = self.gridshape
+        return ((nx + self.dsfx - 1) // self.dsfx, (nz + self.dsfz - 1) // self.dsfz)

@@ -18,7 +18,7 @@ int fail(int code, const char *fmt, ...);
 int check(hipError_t e, const char *what);
 inline int check_launch(const char *what) { return check(hipGetLastError(), what); }
 
-inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Carves aligned sub-buffers out of a caller-provided workspace.
 struct Carver {
@@ -109,5 +109,11 @@ int launch_debug_indices(int ntx, int nrx, const float *ttx, const float *trx, i
 // ------------------------------------------------------------------ DMAS ---
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                 int nlag, int dmastype, int sumlags, float *out, hipStream_t s);
+
+// ------------------------------------------------------------------ SLSC ---
+// lags is a host array; arguments validated by pycbf_slsc.
+int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
+                const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+                int nlag, int precision, float *out, hipStream_t s);
 
 }  // namespace pycbf

@@ -1,0 +1,362 @@
+// Short-lag spatial coherence (SLSC) of per-receive-channel images.
+//
+// Semantics: calc_lagn_coherence and the pixel loop of
+// examples/advanced_beamforming.ipynb:65-72,205-223.  For output pixel
+// (ix, iz) = (ixo*dsfx, izo*dsfz) the axial window is
+// [iz + kofs, iz + kofs + kwin - 1] clipped to the image; an element is
+// selected when its apodisation is non-zero over the whole window; with fewer
+// than max(lag)+1 selected elements the output is 0.  Per lag m the
+// coherence is the mean, over pairs of selected elements m apart in the
+// selected list, of the window cross-covariance over the product of the
+// population standard deviations; a deviation below 1e-10 drops the pairs of
+// that element (numpy nanmean), and a lag with no pair left gives NaN.  The
+// output is the mean over lags.
+//
+// Both kernels normalise each selected channel once per pixel,
+//   z[p][w] = (x[p][w] - mean_p) / (std_p * sqrt(K)),
+// so that the lag-m coherence is S_m / C_m with S_m = sum_p <z_p, z_p+m> and
+// C_m the number of pairs with two valid channels (invalid channels get
+// z = 0).  Normalisation, pair counts and the lag average are FP32.
+//
+//   PREC 0  S_m on the VALU in FP32 (one lane per channel, wave reductions).
+//   PREC 1  z in FP16, S_m from 32x32 Gram blocks on MFMA (FP32 accumulate).
+//   PREC 2  z in BF16, same MFMA scheme.
+//
+// MFMA scheme: with the selected channels in blocks of 32, every pair (p, p+m)
+// with m < 32 lies in a diagonal block (b, b) or its right neighbour
+// (b, b+1); lags 32..63 use (b, b+1) and (b, b+2).  v_mfma_f32_32x32x8
+// leaves G[i][j] of a block in lane (j + 32*h), register r, with
+// i = 8*(r/4) + 4*h + r%4.  Taking G0[i][j] if j >= i else G1[i][j] and
+// rotating each register row left by i lanes (ds_bpermute) lands every entry
+// of lag 32q + m' in lane m', so one add per register accumulates all lags of
+// a block pair at once; no per-pair value leaves the registers.
+#include <math.h>
+
+#include "common.h"
+#include "prof.h"
+
+namespace pycbf {
+
+namespace {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ unsigned lanes_below(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ unsigned short to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (inputs are finite)
+    return (unsigned short)(u >> 16);
+}
+
+// pairs of valid channels m apart (1 <= m <= 63) in a 256-bit mask V0..V3
+__device__ __forceinline__ int pair_count(unsigned long long V0, unsigned long long V1,
+                                          unsigned long long V2, unsigned long long V3, int m)
+{
+    const int l = 64 - m;
+    return __popcll(V0 & ((V0 >> m) | (V1 << l))) + __popcll(V1 & ((V1 >> m) | (V2 << l))) +
+           __popcll(V2 & ((V2 >> m) | (V3 << l))) + __popcll(V3 & (V3 >> m));
+}
+
+}  // namespace
+
+struct SlscArgs {
+    const float *im;    // (nfr, nrx, nx, nz)
+    const float *apod;  // (nrx, nx, nz) or null
+    float *out;         // (nfr, nxo, nzo)
+    int64_t nrx, nx, nz;
+    int nxo, nzo, kwin, kofs, dsfx, dsfz;
+    int pz;        // output pixels per workgroup (one axial strip)
+    int span;      // staged samples per channel
+    int xpitch;    // floats per staged channel row
+    int zbytes;    // bytes of one wave's normalised-channel buffer
+    int nlag, need, ngroups;
+    float inv_nlag;
+    int lags[64];
+};
+
+template <int PREC, int KP>
+__global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nrx = (int)a.nrx;
+    float *xs = reinterpret_cast<float *>(smem);
+    unsigned char *okf = smem + (size_t)nrx * a.xpitch * 4;
+    const size_t zoff = align_up((size_t)nrx * a.xpitch * 4 + (a.apod ? (size_t)nrx * a.span : 0),
+                                 16);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    unsigned char *zbuf = smem + zoff + (size_t)wave * a.zbytes;
+    unsigned char *vflag = smem + zoff + (size_t)nwave * a.zbytes + (size_t)wave * 256;
+
+    const int f = blockIdx.z, ixo = blockIdx.y;
+    const int ix = ixo * a.dsfx;
+    const int izo0 = blockIdx.x * a.pz;
+    const int zs0 = izo0 * a.dsfz + a.kofs;
+    const int64_t rstride = a.nx * a.nz;
+
+    // stage the strip's samples of every channel (zero outside the image)
+    {
+        const float *src = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;
+        const float *asrc = a.apod ? a.apod + (int64_t)ix * a.nz : nullptr;
+        const int tot = nrx * a.span;
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            const int r = t / a.span, zz = t - r * a.span, z = zs0 + zz;
+            const bool in = z >= 0 && (int64_t)z < a.nz;
+            xs[r * a.xpitch + zz] = in ? src[r * rstride + z] : 0.0f;
+            if (asrc) okf[r * a.span + zz] = in && asrc[r * rstride + z] != 0.0f;
+        }
+    }
+    __syncthreads();
+
+    for (int k = wave; k < a.pz; k += nwave) {
+        const int izo = izo0 + k;
+        if (izo >= a.nzo) break;
+        const int iz = izo * a.dsfz;
+        const int w0 = max(iz + a.kofs, 0);
+        const int w1 = min((int64_t)(iz + a.kofs + a.kwin), a.nz);
+        const int keff = w1 - w0, off = w0 - zs0;
+        const float fk = (float)keff;
+        const float sk = sqrtf(fk);
+
+        // select, normalise and compact the channels
+        int n = 0;
+        for (int c = 0; c < nrx; c += 64) {
+            const int r = c + lane;
+            bool sel = r < nrx;
+            if (sel && a.apod)
+                for (int w = 0; w < keff; ++w) sel = sel && okf[r * a.span + off + w];
+            const float *xr = xs + (sel ? r : 0) * a.xpitch + off;
+            float mu = 0.0f, rs = 0.0f;
+            bool valid = false;
+            if (sel) {
+                float s = 0.0f;
+                for (int w = 0; w < keff; ++w) s += xr[w];
+                mu = s / fk;
+                float v = 0.0f;
+                for (int w = 0; w < keff; ++w) {
+                    const float d = xr[w] - mu;
+                    v = __builtin_fmaf(d, d, v);
+                }
+                const float sd = sqrtf(v / fk);
+                valid = sd >= 1e-10f;
+                rs = valid ? 1.0f / (sd * sk) : 0.0f;
+            }
+            const unsigned long long sm = __ballot(sel);
+            const int p = n + (int)lanes_below(sm);
+            if (sel) {
+                vflag[p] = valid ? 1 : 0;
+                if (PREC == 0) {
+                    float *zr = reinterpret_cast<float *>(zbuf) + p * (KP + 1);
+#pragma unroll
+                    for (int w = 0; w < KP; ++w) zr[w] = w < keff ? (xr[w] - mu) * rs : 0.0f;
+                } else {
+                    unsigned short *zr = reinterpret_cast<unsigned short *>(zbuf) + p * KP;
+#pragma unroll
+                    for (int w = 0; w < KP; ++w) {
+                        const float z = w < keff ? (xr[w] - mu) * rs : 0.0f;
+                        if (PREC == 1)
+                            zr[w] = __builtin_bit_cast(unsigned short, (_Float16)z);
+                        else
+                            zr[w] = to_bf16(z);
+                    }
+                }
+            }
+            n += __popcll(sm);
+        }
+
+        float *dst = a.out + ((int64_t)f * a.nxo + ixo) * a.nzo + izo;
+        if (n < a.need) {
+            if (lane == 0) *dst = 0.0f;
+            continue;
+        }
+        // valid-channel bits in selected order
+        unsigned long long V0 = 0ull, V1 = 0ull, V2 = 0ull, V3 = 0ull;
+        V0 = __ballot(lane < n && vflag[lane]);
+        if (n > 64) V1 = __ballot(64 + lane < n && vflag[64 + lane]);
+        if (n > 128) V2 = __ballot(128 + lane < n && vflag[128 + lane]);
+        if (n > 192) V3 = __ballot(192 + lane < n && vflag[192 + lane]);
+
+        float S = 0.0f;  // lane li: S of lag lags[li]
+        if (PREC == 0) {
+            const float *zb = reinterpret_cast<const float *>(zbuf);
+            for (int pc = 0; pc < n; pc += 64) {
+                const int p = pc + lane;
+                const bool act = p < n;
+                float zr[KP];
+#pragma unroll
+                for (int w = 0; w < KP; ++w) zr[w] = act ? zb[p * (KP + 1) + w] : 0.0f;
+                for (int li = 0; li < a.nlag; ++li) {
+                    const int m = a.lags[li];
+                    float s = 0.0f;
+                    if (act && p + m < n) {
+                        const float *zq = zb + (p + m) * (KP + 1);
+#pragma unroll
+                        for (int w = 0; w < KP; ++w) s = __builtin_fmaf(zr[w], zq[w], s);
+                    }
+                    s = wave_sum(s);
+                    if (lane == li) S += s;
+                }
+            }
+        } else {
+            // zero rows n .. 32*nblk-1 of the last block
+            const int nblk = (n + 31) >> 5;
+            unsigned short *zr16 = reinterpret_cast<unsigned short *>(zbuf);
+            for (int t = n * KP + lane; t < nblk * 32 * KP; t += 64) zr16[t] = 0;
+            const int j = lane & 31, h = lane >> 5;
+            const int mylag = lane < a.nlag ? a.lags[lane] : 0;
+            for (int q = 0; q < a.ngroups; ++q) {
+                float acc = 0.0f;
+                for (int b = 0; b + q < nblk; ++b) {
+                    f32x16_t g0 = {}, g1 = {};
+                    const bool two = b + q + 1 < nblk;
+#pragma unroll
+                    for (int kc = 0; kc < KP / 8; ++kc) {
+                        const int ko = kc * 8 + 4 * h;
+                        const unsigned short *pa = zr16 + (32 * b + j) * KP + ko;
+                        const unsigned short *pb0 = zr16 + (32 * (b + q) + j) * KP + ko;
+                        const unsigned short *pb1 = zr16 + (32 * (b + q + 1) + j) * KP + ko;
+                        if (PREC == 1) {
+                            const half4_t va = *reinterpret_cast<const half4_t *>(pa);
+                            g0 = __builtin_amdgcn_mfma_f32_32x32x8f16(
+                                va, *reinterpret_cast<const half4_t *>(pb0), g0, 0, 0, 0);
+                            if (two)
+                                g1 = __builtin_amdgcn_mfma_f32_32x32x8f16(
+                                    va, *reinterpret_cast<const half4_t *>(pb1), g1, 0, 0, 0);
+                        } else {
+                            const bf16x4_t va = *reinterpret_cast<const bf16x4_t *>(pa);
+                            g0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+                                va, *reinterpret_cast<const bf16x4_t *>(pb0), g0, 0, 0, 0);
+                            if (two)
+                                g1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+                                    va, *reinterpret_cast<const bf16x4_t *>(pb1), g1, 0, 0, 0);
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+                        const float v = j >= i ? g0[r] : g1[r];
+                        const int srcl = 32 * h + ((j + i) & 31);
+                        acc += __int_as_float(
+                            __builtin_amdgcn_ds_bpermute(srcl << 2, __float_as_int(v)));
+                    }
+                }
+                acc += __shfl_xor(acc, 32);
+                const float got = __shfl(acc, mylag & 31);
+                if (lane < a.nlag && (mylag >> 5) == q) S = got;
+            }
+        }
+
+        float contrib = 0.0f;
+        if (lane < a.nlag) {
+            const int cnt = pair_count(V0, V1, V2, V3, a.lags[lane]);
+            const float rho = cnt > 0 ? S / (float)cnt : __builtin_nanf("");
+            contrib = rho * a.inv_nlag;
+        }
+        contrib = wave_sum(contrib);
+        if (lane == 0) *dst = contrib;
+    }
+}
+
+// Staging geometry and LDS size for a launch; returns 0 or an error status.
+static int slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
+{
+    const int nrx = (int)a.nrx;
+    const int npad = (nrx + 31) / 32 * 32;
+    a.zbytes = (int)align_up(prec == 0 ? (size_t)npad * (kp + 1) * 4 : (size_t)npad * kp * 2, 16);
+    const size_t budget = 64 * 1024;
+    // strip length: keep the staged channels within about 32 KB
+    int pz = 16;
+    for (; pz > 1; --pz) {
+        const int span = (pz - 1) * a.dsfz + a.kwin;
+        if ((size_t)nrx * (span + 1) * 4 <= 32 * 1024) break;
+    }
+    pz = std::min<int>(pz, a.nzo);
+    for (;;) {
+        a.pz = pz;
+        a.span = (pz - 1) * a.dsfz + a.kwin;
+        a.xpitch = a.span | 1;
+        const size_t stage = align_up((size_t)nrx * a.xpitch * 4 + (a.apod ? (size_t)nrx * a.span : 0),
+                                      16);
+        for (nwave = std::min(4, pz); nwave >= 1; --nwave) {
+            lds = stage + (size_t)nwave * a.zbytes + (size_t)nwave * 256;
+            if (lds <= budget) return 0;
+        }
+        if (pz == 1) break;
+        pz = std::max(1, pz / 2);
+    }
+    return fail(PYCBF_EINVAL, "SLSC of %d channels with a %d-sample window does not fit in LDS",
+                nrx, a.kwin);
+}
+
+template <int PREC, int KP>
+static int launch_slsc_t(const SlscArgs &a, int nwave, size_t lds, int64_t nfr, hipStream_t s)
+{
+    const dim3 grid((unsigned)((a.nzo + a.pz - 1) / a.pz), (unsigned)a.nxo, (unsigned)nfr);
+    ProfScope ps(PREC == 0 ? "slsc_fp32" : (PREC == 1 ? "slsc_mfma_fp16" : "slsc_mfma_bf16"), s);
+    hipLaunchKernelGGL((k_slsc<PREC, KP>), grid, dim3(64 * nwave), lds, s, a);
+    return check_launch("slsc");
+}
+
+template <int PREC>
+static int launch_slsc_p(const SlscArgs &a, int kp, int nwave, size_t lds, int64_t nfr,
+                         hipStream_t s)
+{
+    switch (kp) {
+    case 8: return launch_slsc_t<PREC, 8>(a, nwave, lds, nfr, s);
+    case 16: return launch_slsc_t<PREC, 16>(a, nwave, lds, nfr, s);
+    default: return launch_slsc_t<PREC, 32>(a, nwave, lds, nfr, s);
+    }
+}
+
+int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
+                const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+                int nlag, int precision, float *out, hipStream_t s)
+{
+    SlscArgs a{};
+    a.im = im;
+    a.apod = apod;
+    a.out = out;
+    a.nrx = nrx;
+    a.nx = nx;
+    a.nz = nz;
+    a.kwin = kwin;
+    a.kofs = kofs;
+    a.dsfx = dsfx;
+    a.dsfz = dsfz;
+    a.nxo = (int)((nx + dsfx - 1) / dsfx);
+    a.nzo = (int)((nz + dsfz - 1) / dsfz);
+    a.nlag = nlag;
+    int lagmax = 0;
+    for (int i = 0; i < nlag; ++i) {
+        a.lags[i] = lags[i];
+        lagmax = std::max(lagmax, (int)lags[i]);
+    }
+    a.need = lagmax + 1;
+    a.ngroups = lagmax / 32 + 1;
+    a.inv_nlag = 1.0f / (float)nlag;
+    const int kp = kwin <= 8 ? 8 : (kwin <= 16 ? 16 : 32);
+    int nwave = 0;
+    size_t lds = 0;
+    if (int rc = slsc_plan(a, precision, kp, nwave, lds)) return rc;
+    switch (precision) {
+    case PYCBF_SLSC_FP32: return launch_slsc_p<0>(a, kp, nwave, lds, nfr, s);
+    case PYCBF_SLSC_MFMA_FP16: return launch_slsc_p<1>(a, kp, nwave, lds, nfr, s);
+    default: return launch_slsc_p<2>(a, kp, nwave, lds, nfr, s);
+    }
+}
+
+}  // namespace pycbf

@@ -33,6 +33,7 @@ lib = ct.CDLL(LIBPATH)
 INTERP = {"nearest": 0, "linear": 1, "akima": 2, "makima": 3, "korder_cubic": 4, "cubic": 5}
 SUMTYPE = {"none": 0, "tx_only": 1, "rx_only": 2, "tx_and_rx": 3}
 DMASTYPE = {"ssr": 0, "power": 1, "pssr": 2}
+SLSC_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 TXPACK_FLOATS = 16
 RXPACK_FLOATS = 8
 ABI_VERSION = 1
@@ -77,6 +78,8 @@ _SIGS = {
     "pycbf_debug_indices": ([_prf, _vp, _vp, _i64, ct.c_int, ct.c_int, _vp, _vp], ct.c_int),
     "pycbf_dmas": ([_vp, _i64, _i64, _i64, _vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp],
                    ct.c_int),
+    "pycbf_slsc": ([_vp, _i64, _i64, _i64, _i64, _vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                    _vp, ct.c_int, ct.c_int, _vp, _vp], ct.c_int),
     "pycbf_profile_enable": ([ct.c_int], None),
     "pycbf_profile_reset": ([], None),
     "pycbf_profile_read": ([ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)],

@@ -1,4 +1,5 @@
-"""GPU DMAS beamformers (drop-in for pycbf/gpu/__advanced_bf_classes__.py).
+"""GPU DMAS and SLSC beamformers (drop-in for pycbf/gpu/__advanced_bf_classes__.py,
+plus SLSC, which the reference only has as a notebook loop).
 
 The base DAS runs with sumtype 'tx_only' (lagaxis 'rx') or 'rx_only' (lagaxis
 'tx'), then pycbf_dmas correlates the separated images at the requested lags.
@@ -10,7 +11,9 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
-from ..base import BeamformerException, DMASBeamformer
+import numpy as np
+
+from ..base import BeamformerException, DMASBeamformer, SLSCBeamformer
 from . import _engine as E
 from .core import SyntheticBeamformer, TabbedBeamformer, _torch
 
@@ -70,3 +73,92 @@ class SyntheticDMASBeamformer(SyntheticBeamformer, DMASBeamformer, _GPUDMAS):
 
     def __call__(self, txrxt, out_as_numpy: bool = True, buffer=None):
         return self._dmas_call(SyntheticBeamformer.__call__, txrxt, out_as_numpy, buffer)
+
+
+def slsc(im, lags, kwin, kofs=None, apod=None, dsfx=1, dsfz=1, precision="fp16", out=None):
+    """SLSC of per-receive-channel images on the GPU (pycbf_slsc).
+
+    im: float32 CUDA tensor (nrx, nx, nz) or (nfr, nrx, nx, nz); apod: None or
+    (nrx, nx, nz); returns (nfr?, ceil(nx/dsfx), ceil(nz/dsfz)) float32."""
+    torch = _torch()
+    if not (isinstance(im, torch.Tensor) and im.is_cuda and im.dtype == torch.float32):
+        raise BeamformerException("im must be a float32 CUDA tensor")
+    squeeze = im.dim() == 3
+    im4 = (im[None] if squeeze else im).contiguous()
+    if im4.dim() != 4:
+        raise BeamformerException("im must be (nrx, nx, nz) or (nfr, nrx, nx, nz)")
+    nfr, nrx, nx, nz = im4.shape
+    kofs = -(int(kwin) // 2) if kofs is None else int(kofs)
+    lags_h = np.ascontiguousarray(np.asarray(lags).ravel(), np.int32)
+    if apod is not None:
+        apod = apod.to(device=im.device, dtype=torch.float32).contiguous()
+        if tuple(apod.shape) != (nrx, nx, nz):
+            raise BeamformerException(f"apod must be {(nrx, nx, nz)}")
+    shape = (nfr, (nx + dsfx - 1) // dsfx, (nz + dsfz - 1) // dsfz)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=im.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        raise BeamformerException(f"out must be a contiguous float32 tensor of shape {shape}")
+    rc = E.lib.pycbf_slsc(E.ptr(im4), nfr, nrx, nx, nz, E.ptr(apod), int(kwin), kofs, int(dsfx),
+                          int(dsfz), lags_h.ctypes.data, lags_h.size,
+                          E.SLSC_PRECISION[precision], E.ptr(out), E.stream_of(im.device))
+    E.check(rc, "pycbf_slsc")
+    return out[0] if squeeze else out
+
+
+class _GPUSLSC:
+    def _slsc_setup(self, apodrx):
+        nx, nz = self.gridshape
+        if nx * nz != self.nop:
+            raise BeamformerException(
+                f"gridshape {self.gridshape} holds {nx * nz} pixels but there are {self.nop}")
+        self._slsc_apod = apodrx.reshape(self.nrx, nx, nz).contiguous()
+
+    def _slsc_call(self, base_call, txrxt, out_as_numpy, buffer):
+        torch = _torch()
+        rf, nframes = self._coerce_rf(txrxt)
+        imrx = base_call(self, rf, out_as_numpy=False)
+        nf = 1 if nframes is None else nframes
+        nx, nz = self.gridshape
+        shape = self._slsc_outshape() if nframes is None else (nframes,) + self._slsc_outshape()
+        if buffer is None:
+            out = torch.empty(shape, dtype=torch.float32, device=self._dev)
+        else:
+            out = buffer
+            if not (isinstance(out, torch.Tensor) and tuple(out.shape) == shape
+                    and out.dtype == torch.float32 and out.device == self._dev
+                    and out.is_contiguous()):
+                raise BeamformerException(
+                    f"buffer must be a contiguous float32 tensor of shape {shape} on {self._dev}")
+        slsc(imrx.reshape(nf, self.nrx, nx, nz), self.lags, self.kwin, self.kofs,
+             self._slsc_apod, self.dsfx, self.dsfz, self.precision,
+             out=out.reshape((nf,) + self._slsc_outshape()))
+        return out.cpu().numpy() if out_as_numpy else out
+
+
+@dataclass(kw_only=True)
+class TabbedSLSCBeamformer(TabbedBeamformer, SLSCBeamformer, _GPUSLSC):
+    """Short-lag spatial coherence over precomputed delay tabs; elements are
+    selected by ``apodrx``."""
+
+    def __post_init__(self):
+        SLSCBeamformer.__post_init__(self)
+        TabbedBeamformer.__post_init__(self)
+        self._slsc_setup(self._apodrx)
+
+    def __call__(self, txrxt, out_as_numpy: bool = True, buffer=None):
+        return self._slsc_call(TabbedBeamformer.__call__, txrxt, out_as_numpy, buffer)
+
+
+@dataclass(kw_only=True)
+class SyntheticSLSCBeamformer(SyntheticBeamformer, SLSCBeamformer, _GPUSLSC):
+    """Short-lag spatial coherence over synthetic-point geometry; elements are
+    selected by the receive acceptance apodisation of the geometry."""
+
+    def __post_init__(self):
+        SLSCBeamformer.__post_init__(self)
+        SyntheticBeamformer.__post_init__(self)
+        self._slsc_setup(self.tables()[3])
+
+    def __call__(self, txrxt, out_as_numpy: bool = True, buffer=None):
+        return self._slsc_call(SyntheticBeamformer.__call__, txrxt, out_as_numpy, buffer)

@@ -286,3 +286,67 @@ def test_slsc_hand_computed():
     r2 = corr(X[:, 0, 0], X[:, 2, 0])
     got = numerics.slsc(X, np.ones((3, 1), bool), [1, 2])[0]
     assert abs(got - 0.5 * (r1 + r2)) < 1e-12
+
+
+def test_slsc_image_closed_forms():
+    """The windowed pixel loop: coherent channels give 1, alternating signs -1
+    at lag 1, output 0 without enough elements, and the output grid follows
+    the downsampling factors."""
+    rng = np.random.default_rng(16)
+    nrx, nx, nz = 12, 3, 20
+    base = rng.standard_normal((nx, nz))
+    gain = rng.uniform(0.5, 2.0, nrx)
+    im = gain[:, None, None] * base[None]
+    out = numerics.slsc_image(im, None, [1, 2], kwin=5, kofs=-2)
+    assert out.shape == (nx, nz)
+    np.testing.assert_allclose(out, 1.0, atol=1e-12)
+    alt = im * ((-1.0) ** np.arange(nrx))[:, None, None]
+    np.testing.assert_allclose(numerics.slsc_image(alt, None, [1], 5, -2), -1.0, atol=1e-12)
+    np.testing.assert_allclose(numerics.slsc_image(alt, None, [1, 2], 5, -2), 0.0, atol=1e-12)
+    assert numerics.slsc_image(im, None, [11, 12], 5, -2).max() == 0.0   # need 13 elements
+    ds = numerics.slsc_image(im, None, [1], 4, -1, dsfx=2, dsfz=3)
+    assert ds.shape == (2, 7)
+
+
+def test_slsc_image_pairs_follow_the_selected_list():
+    """Lag m pairs elements m apart in the list of selected elements: with
+    every other element deselected, alternating-sign data becomes coherent."""
+    rng = np.random.default_rng(17)
+    nrx, nx, nz = 16, 2, 12
+    base = rng.standard_normal((nx, nz))
+    alt = ((-1.0) ** np.arange(nrx))[:, None, None] * base[None]
+    apod = np.ones((nrx, nx, nz))
+    apod[1::2] = 0.0                        # keep the even (same-sign) elements
+    np.testing.assert_allclose(numerics.slsc_image(alt, apod, [1], 3, -1), 1.0, atol=1e-12)
+    # an element with a single zero apodisation sample inside the window drops out
+    apod2 = np.ones((nrx, nx, nz))
+    apod2[1::2, :, 6] = 0.0
+    got = numerics.slsc_image(alt, apod2, [1], 3, -1)
+    assert np.allclose(got[:, 5:8], 1.0) and np.allclose(got[:, :4], -1.0)
+
+
+def test_slsc_image_window_is_clipped_at_the_edges():
+    """At the first and last pixels only the samples inside the image count:
+    channels coherent on z 0..2 and z nz-3..nz-1 but not elsewhere give 1 there
+    only if the window is clipped."""
+    rng = np.random.default_rng(18)
+    nrx, nz = 10, 14
+    im = rng.standard_normal((nrx, 1, nz))
+    shared = rng.standard_normal(nz)
+    for z in (0, 1, 2, nz - 3, nz - 2, nz - 1):
+        im[:, 0, z] = shared[z] * np.linspace(1, 2, nrx)
+    out = numerics.slsc_image(im, None, [1, 3], kwin=5, kofs=-2)
+    assert abs(out[0, 0] - 1.0) < 1e-12 and abs(out[0, nz - 1] - 1.0) < 1e-12
+    assert abs(out[0, nz // 2]) < 0.9
+
+
+def test_slsc_image_zero_variance_channels_drop_their_pairs():
+    rng = np.random.default_rng(19)
+    nrx, nz = 10, 9
+    base = rng.standard_normal(nz)
+    im = np.tile(base, (nrx, 1))[:, None, :] * np.linspace(1, 3, nrx)[:, None, None]
+    im[4] = 2.5                              # constant channel: std 0
+    out = numerics.slsc_image(im, None, [1], 3, -1)
+    np.testing.assert_allclose(out, 1.0, atol=1e-12)   # pairs (3,4), (4,5) dropped
+    im[::2] = 1.0                            # every lag-1 pair now has a flat member
+    assert np.all(np.isnan(numerics.slsc_image(im, None, [1], 3, -1)))
