@@ -79,11 +79,12 @@ def measured_traffic(frames):
     return vals["traffic_bytes_per_launch"]
 
 
-def cpu_baseline(rf0, params, stride, threads):
+def cpu_baseline(frames, params, stride, threads):
     """Reference C cubic DAS (oracle/_ref, pycbf/cpu/pycbf.c) -- or the oracle
-    restatement when the reference build is absent -- on frame 0 over a strided
-    pixel subset, channels split over `threads` host threads with private
-    accumulators.  Returns (Gsamples/s, threads, kind, sample description)."""
+    restatement when the reference build is absent -- on the given host frames
+    over a strided pixel subset, channels split over `threads` host threads
+    with private accumulators.  Returns (Gsamples/s, threads, kind, sample
+    description)."""
     import ctypes as ct
 
     import oracle
@@ -97,12 +98,11 @@ def cpu_baseline(rf0, params, stride, threads):
     kind = "reference" if R is not None else "port"
     fn = R.beamform_cubic if R is not None else oracle.lib().orc_bf_cubic
     pf = ct.POINTER(ct.c_float)
-    rf0 = np.ascontiguousarray(rf0, np.float32)
     outs = [np.zeros(npx, np.float32) for _ in range(threads)]
     dt = float(np.float32(1.0 / 40e6))
     chans = [(t, r) for t in range(NTX) for r in range(NRX)]
 
-    def work(k):
+    def work(k, rf0):
         o = outs[k].ctypes.data_as(pf)
         for idx in range(k, len(chans), threads):
             t, r = chans[idx]
@@ -110,17 +110,103 @@ def cpu_baseline(rf0, params, stride, threads):
                tautx[t].ctypes.data_as(pf), apodtx[t].ctypes.data_as(pf),
                taurx[r].ctypes.data_as(pf), apodrx[r].ctypes.data_as(pf), o)
 
-    ths = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
-    t0 = time.perf_counter()
-    for th in ths:
-        th.start()
-    for th in ths:
-        th.join()
-    wall = time.perf_counter() - t0
-    samples = NTX * NRX * npx
-    desc = (f"frame 0, {NTX}x{NRX} channels, every {stride}th of {NX * NZ} pixels "
-            f"({npx} px, {samples / 1e6:.0f} M samples), cubic, {wall:.2f} s wall")
+    wall = 0.0
+    for rf0 in frames:
+        rf0 = np.ascontiguousarray(rf0, np.float32)
+        ths = [threading.Thread(target=work, args=(k, rf0)) for k in range(threads)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        wall += time.perf_counter() - t0
+    samples = NTX * NRX * npx * len(frames)
+    desc = (f"{len(frames)} frames, {NTX}x{NRX} channels, every {stride}th of {NX * NZ} pixels "
+            f"({npx} px, {samples / 1e9:.1f} G samples), cubic, {wall:.2f} s wall")
     return samples / wall / 1e9, threads, kind, desc
+
+
+SLSC_NXO, SLSC_NZO, SLSC_SUB, SLSC_LAGS, SLSC_KWIN = 256, 256, 8, list(range(1, 26)), 8
+
+
+def slsc_leg(dev, reps, with_cpu):
+    """Coherence workload: 75 plane waves x 128 rx beamformed per receive
+    channel (cubic, transmits summed) on a 256 x 2048 fine grid over 40 mm of
+    depth (speckle with a 5 mm anechoic cyst), then SLSC with lags 1..25 and
+    an 8-sample axial kernel onto the 256 x 256 output grid, in FP32 and on
+    MFMA with FP16 / BF16 inputs.  Returns the JSON object of the leg."""
+    import torch
+
+    from pycbf_amd import sim
+    from pycbf_amd.gpu import SyntheticBeamformer, slsc
+    from pycbf_amd.gpu import _engine as E
+    steers, xele, params, (nx, nzf) = sim.workload_slsc(NRX, NTX, SLSC_NXO, SLSC_NZO, SLSC_SUB)
+    xs, zs, amp = sim.cyst_speckle(20_000, (-19e-3, 19e-3), (2e-3, 42e-3), (0.0, 22e-3), 2.5e-3,
+                                   4321)
+    rf = sim.simulate_rf_torch(steers, xele, NT, xs, zs, amp, dev, seed=4321)
+    bf = SyntheticBeamformer(**params, t0=0.0, dt=1.0 / 40e6, nt=NT, interp={"kind": "cubic"},
+                             sumtype="tx_only", device=dev)
+    imrx = torch.empty((NRX, nx * nzf), dtype=torch.float32, device=dev)
+    bf(rf, out_as_numpy=False, buffer=imrx)
+    torch.cuda.synchronize()
+    das_reps = 3
+    t0 = time.perf_counter()
+    for _ in range(das_reps):
+        bf(rf, out_as_numpy=False, buffer=imrx)
+    torch.cuda.synchronize()
+    das_ms = (time.perf_counter() - t0) * 1e3 / das_reps
+    log(f"slsc leg: per-rx DAS {NTX}x{NRX}x{nx}x{nzf} {das_ms:.3f} ms")
+    im4 = imrx.view(1, NRX, nx, nzf)
+    from pycbf_amd.gpu import slsc_selection
+    sel = slsc_selection(bf.tables()[3].view(NRX, nx, nzf), SLSC_KWIN, -SLSC_KWIN // 2, 1,
+                         SLSC_SUB, device=dev)
+    out = torch.empty((1, SLSC_NXO, SLSC_NZO), dtype=torch.float32, device=dev)
+    names = {"fp32": "slsc_fp32", "fp16": "slsc_mfma_fp16", "bf16": "slsc_mfma_bf16"}
+    ms = {}
+    for prec, kname in names.items():
+        slsc(im4, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, None, 1, SLSC_SUB, prec, out=out, sel=sel)
+        torch.cuda.synchronize()
+        E.lib.pycbf_profile_reset()
+        E.lib.pycbf_profile_enable(1)
+        for _ in range(reps):
+            slsc(im4, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, None, 1, SLSC_SUB, prec, out=out, sel=sel)
+        torch.cuda.synchronize()
+        E.lib.pycbf_profile_enable(0)
+        k_ms, k_n = E.profile_read(kname)
+        ms[prec] = k_ms / max(k_n, 1)
+        log(f"slsc {prec}: {ms[prec] * 1e3:.1f} us/launch")
+    npx = SLSC_NXO * SLSC_NZO
+    algo = 4.0 * NRX * nx * nzf + 36.0 * npx     # per-rx images + selection bits + output
+    res = {
+        "workload": f"SLSC lags 1..25, 8-sample axial kernel, {NRX} rx x {NTX} PW, "
+                    f"{SLSC_NXO}x{SLSC_NZO} px from a {nx}x{nzf} per-rx grid (cyst phantom)",
+        "px_per_s": {p: round(npx / (ms[p] * 1e-3), 1) for p in ms},
+        "kernel_us": {p: round(ms[p] * 1e3, 2) for p in ms},
+        "mfma_fp16_speedup_vs_fp32": round(ms["fp32"] / ms["fp16"], 3),
+        "mfma_bf16_speedup_vs_fp32": round(ms["fp32"] / ms["bf16"], 3),
+        "roofline_fp16": {"bound": "hbm", "achieved": round(algo / (ms["fp16"] * 1e-3) / 1e9, 1),
+                          "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                          "frac": round(algo / (ms["fp16"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+        "per_rx_das_ms": round(das_ms, 3),
+        "pipeline_px_per_s": round(npx / ((das_ms + ms["fp16"]) * 1e-3), 1),
+    }
+    if with_cpu:
+        from oracle import numerics
+        cols = list(range(0, nx, 16))
+        sub = imrx.view(NRX, nx, nzf)[:, cols].cpu().numpy()
+        asub = bf.tables()[3].view(NRX, nx, nzf)[:, cols].cpu().numpy()
+        t0 = time.perf_counter()
+        numerics.slsc_image(sub, asub, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, 1, SLSC_SUB)
+        wall = time.perf_counter() - t0
+        n = len(cols) * SLSC_NZO
+        res["cpu_baseline"] = {"value": round(n / wall, 1), "unit": "px/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"notebook pixel loop (numpy FP64) on {len(cols)} of "
+                                         f"{SLSC_NXO} columns ({n} px), {wall:.2f} s"}
+        log(f"slsc cpu baseline: {n / wall:.1f} px/s")
+    del rf, imrx, sel
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -130,7 +216,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=2)
+    ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--cpu-frames", type=int, default=12,
+                    help="frames timed for the CPU baseline (about 10-30 s of host work)")
+    ap.add_argument("--no-slsc", action="store_true", help="skip the coherence leg")
+    ap.add_argument("--slsc-reps", type=int, default=20)
     ap.add_argument("--data", choices=["sim", "noise"], default="sim",
                     help="noise: seeded Gaussian RF of the same shape (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -202,11 +292,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        rf0 = rf[0].cpu().numpy()
-        rate, used, kind, desc = cpu_baseline(rf0, params, args.cpu_stride, threads)
+        frames = (rf[f].cpu().numpy() for f in range(min(args.cpu_frames, args.frames)))
+        rate, used, kind, desc = cpu_baseline(list(frames), params, args.cpu_stride, threads)
         cpu = {"value": round(rate, 4), "unit": "Gsamples/s", "cores": used, "kind": kind,
                "sample": desc}
         log(f"cpu baseline: {rate:.4f} Gsamples/s on {used} threads ({kind})")
+
+    slsc_res = None
+    if not args.no_slsc:
+        del rf
+        torch.cuda.empty_cache()
+        slsc_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -233,6 +329,7 @@ def main():
         "phases_ms": {"das_gather": round(gather_ms, 4),
                       "spline_coefs": round(s_ms / max(s_n, 1), 4),
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
+        "slsc": slsc_res,
     }
     print(json.dumps(line), flush=True)
 

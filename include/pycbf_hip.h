@@ -175,22 +175,32 @@ int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int
  * GPU SLSC).
  *   im    (nfr, nrx, nx, nz) per-receive-channel images (DAS with sumtype
  *         tx_only over an x-major grid of nx * nz pixels)
- *   apod  (nrx, nx, nz) receive apodisation, or NULL (every element selected)
+ *   sel   (ceil(nx/dsfx), ceil(nz/dsfz), 4) words from pycbf_slsc_selection,
+ *         or NULL (every element usable at every pixel)
  *   out   (nfr, ceil(nx/dsfx), ceil(nz/dsfz)); output (ixo, izo) is pixel
  *         (ixo*dsfx, izo*dsfz) with axial window [iz+kofs, iz+kofs+kwin-1]
  *         clipped to the image.  The notebook's window iz-nkern..iz+nkern is
  *         kwin = 2*nkern+1, kofs = -nkern.
- * An element is used when apod is non-zero over the whole window; with fewer
- * than max(lags)+1 such elements the output is 0.  Per lag, the pair-normalised
- * window correlation is averaged over pairs of used elements that lag apart
- * (pairs of an element with standard deviation < 1e-10 are dropped; a lag with
- * no pair left is NaN); the output is the mean over lags.
+ * With fewer than max(lags)+1 usable elements the output is 0.  Per lag, the
+ * pair-normalised window correlation is averaged over pairs of usable elements
+ * that lag apart in the list of usable elements (pairs of an element with
+ * standard deviation < 1e-10 are dropped; a lag with no pair left is NaN); the
+ * output is the mean over lags.
  *   lags  HOST array of nlag (1..64) lag values in 1..63
  *   limits: 1 <= kwin <= 32, -(kwin-1) <= kofs <= 0, nrx <= 256, dsfx, dsfz >= 1
  *   precision  PYCBF_SLSC_FP32 / _MFMA_FP16 / _MFMA_BF16 */
 int pycbf_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
-               const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+               const uint64_t *sel, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
                int nlag, int precision, float *out, void *stream);
+
+/* Usable elements per SLSC output pixel (the notebook's element selection,
+ * examples/advanced_beamforming.ipynb:211-214): bit r of word r/64 of
+ * sel[ixo, izo] is set when mask[r, ix, z] is non-zero for every z of the
+ * pixel's window.  mask (nrx, nx, nz) bytes (non-zero receive apodisation);
+ * window arguments as pycbf_slsc.  The selection depends on the geometry
+ * only, so it is computed once and reused for every frame. */
+int pycbf_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,
+                         int kofs, int dsfx, int dsfz, uint64_t *sel, void *stream);
 
 /* Per-kernel timing: when enabled, instrumented launches record HIP events on
  * their stream; pycbf_profile_read synchronises them and returns the summed

@@ -489,29 +489,47 @@ int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int
     return launch_dmas(im, nfr, nsep, np, lags, nlag, dmastype, sumlags, out, (hipStream_t)stream);
 }
 
-int pycbf_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
-               const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
-               int nlag, int precision, float *out, void *stream)
+static int slsc_check_window(int64_t nrx, int64_t nx, int64_t nz, int kwin, int kofs, int dsfx,
+                             int dsfz)
 {
-    if (!im || !lags || !out) return fail(PYCBF_EINVAL, "NULL pointer");
-    if (nfr < 1 || nrx < 1 || nx < 1 || nz < 1) return fail(PYCBF_EINVAL, "empty SLSC problem");
+    if (nrx < 1 || nx < 1 || nz < 1) return fail(PYCBF_EINVAL, "empty SLSC problem");
     if (nrx > 256) return fail(PYCBF_EINVAL, "SLSC supports at most 256 channels (got %lld)",
                                (long long)nrx);
     if (kwin < 1 || kwin > 32) return fail(PYCBF_EINVAL, "kwin must be in 1..32 (got %d)", kwin);
     if (kofs > 0 || kofs < -(kwin - 1))
         return fail(PYCBF_EINVAL, "kofs must be in -(kwin-1)..0 (got %d)", kofs);
     if (dsfx < 1 || dsfz < 1) return fail(PYCBF_EINVAL, "dsfx and dsfz must be >= 1");
+    if ((nx + dsfx - 1) / dsfx > 65535)
+        return fail(PYCBF_EINVAL, "SLSC grid too large (nx/dsfx must be <= 65535)");
+    if ((nz + dsfz - 1) / dsfz > 0x7fffffffLL || nx * nz > (1LL << 40))
+        return fail(PYCBF_EINVAL, "SLSC image too large");
+    return 0;
+}
+
+int pycbf_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,
+                         int kofs, int dsfx, int dsfz, uint64_t *sel, void *stream)
+{
+    if (!mask || !sel) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (int rc = slsc_check_window(nrx, nx, nz, kwin, kofs, dsfx, dsfz)) return rc;
+    return launch_slsc_selection(mask, nrx, nx, nz, kwin, kofs, dsfx, dsfz, sel,
+                                 (hipStream_t)stream);
+}
+
+int pycbf_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
+               const uint64_t *sel, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+               int nlag, int precision, float *out, void *stream)
+{
+    if (!im || !lags || !out) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (nfr < 1) return fail(PYCBF_EINVAL, "empty SLSC problem");
+    if (int rc = slsc_check_window(nrx, nx, nz, kwin, kofs, dsfx, dsfz)) return rc;
     if (nlag < 1 || nlag > 64) return fail(PYCBF_EINVAL, "nlag must be in 1..64 (got %d)", nlag);
     for (int i = 0; i < nlag; ++i)
         if (lags[i] < 1 || lags[i] > 63)
             return fail(PYCBF_EINVAL, "lag %d out of range 1..63", (int)lags[i]);
     if (precision < PYCBF_SLSC_FP32 || precision > PYCBF_SLSC_MFMA_BF16)
         return fail(PYCBF_EINVAL, "precision %d out of range", precision);
-    if ((nx + dsfx - 1) / dsfx > 65535 || nfr > 65535)
-        return fail(PYCBF_EINVAL, "SLSC grid too large (nx/dsfx and nfr must be <= 65535)");
-    if ((nz + dsfz - 1) / dsfz > 0x7fffffffLL || nx * nz > (1LL << 40))
-        return fail(PYCBF_EINVAL, "SLSC image too large");
-    return launch_slsc(im, nfr, nrx, nx, nz, apod, kwin, kofs, dsfx, dsfz, lags, nlag, precision,
+    if (nfr > 65535) return fail(PYCBF_EINVAL, "nfr must be <= 65535");
+    return launch_slsc(im, nfr, nrx, nx, nz, sel, kwin, kofs, dsfx, dsfz, lags, nlag, precision,
                        out, (hipStream_t)stream);
 }
 

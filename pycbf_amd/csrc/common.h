@@ -112,8 +112,10 @@ int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const in
 
 // ------------------------------------------------------------------ SLSC ---
 // lags is a host array; arguments validated by pycbf_slsc.
+int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,
+                          int kofs, int dsfx, int dsfz, uint64_t *sel, hipStream_t s);
 int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
-                const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+                const uint64_t *sel, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
                 int nlag, int precision, float *out, hipStream_t s);
 
 }  // namespace pycbf

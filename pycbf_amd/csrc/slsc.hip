@@ -31,6 +31,7 @@
 // of lag 32q + m' in lane m', so one add per register accumulates all lags of
 // a block pair at once; no per-pair value leaves the registers.
 #include <math.h>
+#include <stdlib.h>
 
 #include "common.h"
 #include "prof.h"
@@ -76,7 +77,7 @@ __device__ __forceinline__ int pair_count(unsigned long long V0, unsigned long l
 
 struct SlscArgs {
     const float *im;    // (nfr, nrx, nx, nz)
-    const float *apod;  // (nrx, nx, nz) or null
+    const unsigned long long *sel;  // (nxo, nzo, 4) usable-element bits, or null
     float *out;         // (nfr, nxo, nzo)
     int64_t nrx, nx, nz;
     int nxo, nzo, kwin, kofs, dsfx, dsfz;
@@ -84,6 +85,7 @@ struct SlscArgs {
     int span;      // staged samples per channel
     int xpitch;    // floats per staged channel row
     int zbytes;    // bytes of one wave's normalised-channel buffer
+    int wbytes;    // bytes of one wave's LDS region (channels, flags, reduction rows)
     int nlag, need, ngroups;
     float inv_nlag;
     int lags[64];
@@ -95,12 +97,11 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
     extern __shared__ __align__(16) unsigned char smem[];
     const int nrx = (int)a.nrx;
     float *xs = reinterpret_cast<float *>(smem);
-    unsigned char *okf = smem + (size_t)nrx * a.xpitch * 4;
-    const size_t zoff = align_up((size_t)nrx * a.xpitch * 4 + (a.apod ? (size_t)nrx * a.span : 0),
-                                 16);
+    const size_t zoff = align_up((size_t)nrx * a.xpitch * 4, 16);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    unsigned char *zbuf = smem + zoff + (size_t)wave * a.zbytes;
-    unsigned char *vflag = smem + zoff + (size_t)nwave * a.zbytes + (size_t)wave * 256;
+    unsigned char *zbuf = smem + zoff + (size_t)wave * a.wbytes;
+    unsigned char *vflag = zbuf + a.zbytes;
+    float *red = reinterpret_cast<float *>(vflag + 256);
 
     const int f = blockIdx.z, ixo = blockIdx.y;
     const int ix = ixo * a.dsfx;
@@ -108,16 +109,28 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
     const int zs0 = izo0 * a.dsfz + a.kofs;
     const int64_t rstride = a.nx * a.nz;
 
-    // stage the strip's samples of every channel (zero outside the image)
+    // stage the strip's samples of every channel (zero outside the image): a
+    // wave takes every nwave-th channel, lanes run along z, and the loads of
+    // SB channels are in flight before any is stored
     {
+        constexpr int SB = 16;
         const float *src = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;
-        const float *asrc = a.apod ? a.apod + (int64_t)ix * a.nz : nullptr;
-        const int tot = nrx * a.span;
-        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-            const int r = t / a.span, zz = t - r * a.span, z = zs0 + zz;
-            const bool in = z >= 0 && (int64_t)z < a.nz;
-            xs[r * a.xpitch + zz] = in ? src[r * rstride + z] : 0.0f;
-            if (asrc) okf[r * a.span + zz] = in && asrc[r * rstride + z] != 0.0f;
+        for (int zc = 0; zc < a.span; zc += 64) {
+            const int zz = zc + lane, z = zs0 + zz;
+            const bool in = zz < a.span && z >= 0 && (int64_t)z < a.nz;
+            for (int r0 = wave; r0 < nrx; r0 += nwave * SB) {
+                float v[SB];
+#pragma unroll
+                for (int i = 0; i < SB; ++i) {
+                    const int r = r0 + i * nwave;
+                    v[i] = (in && r < nrx) ? src[r * rstride + z] : 0.0f;
+                }
+#pragma unroll
+                for (int i = 0; i < SB; ++i) {
+                    const int r = r0 + i * nwave;
+                    if (r < nrx && zz < a.span) xs[r * a.xpitch + zz] = v[i];
+                }
+            }
         }
     }
     __syncthreads();
@@ -129,31 +142,38 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
         const int w0 = max(iz + a.kofs, 0);
         const int w1 = min((int64_t)(iz + a.kofs + a.kwin), a.nz);
         const int keff = w1 - w0, off = w0 - zs0;
-        const float fk = (float)keff;
-        const float sk = sqrtf(fk);
+        const float rk = 1.0f / (float)keff;
+        // |std| >= 1e-10  <=>  sum of squared deviations >= 1e-20 * K
+        const float vmin = 1e-20f * (float)keff;
 
         // select, normalise and compact the channels
+        const unsigned long long *pbits =
+            a.sel ? a.sel + ((int64_t)ixo * a.nzo + izo) * 4 : nullptr;
         int n = 0;
         for (int c = 0; c < nrx; c += 64) {
             const int r = c + lane;
             bool sel = r < nrx;
-            if (sel && a.apod)
-                for (int w = 0; w < keff; ++w) sel = sel && okf[r * a.span + off + w];
+            if (pbits) sel = sel && ((pbits[c >> 6] >> lane) & 1ull);
             const float *xr = xs + (sel ? r : 0) * a.xpitch + off;
+            float xv[KP];
+#pragma unroll
+            for (int w = 0; w < KP; ++w) xv[w] = (sel && w < keff) ? xr[w] : 0.0f;
             float mu = 0.0f, rs = 0.0f;
             bool valid = false;
             if (sel) {
                 float s = 0.0f;
-                for (int w = 0; w < keff; ++w) s += xr[w];
-                mu = s / fk;
+#pragma unroll
+                for (int w = 0; w < KP; ++w) s += xv[w];
+                mu = s * rk;
                 float v = 0.0f;
-                for (int w = 0; w < keff; ++w) {
-                    const float d = xr[w] - mu;
+#pragma unroll
+                for (int w = 0; w < KP; ++w) {
+                    const float d = w < keff ? xv[w] - mu : 0.0f;
                     v = __builtin_fmaf(d, d, v);
                 }
-                const float sd = sqrtf(v / fk);
-                valid = sd >= 1e-10f;
-                rs = valid ? 1.0f / (sd * sk) : 0.0f;
+                // z = (x - mean) / (std * sqrt(K)) = (x - mean) / sqrt(sum d^2)
+                valid = v >= vmin;
+                rs = valid ? __frsqrt_rn(v) : 0.0f;
             }
             const unsigned long long sm = __ballot(sel);
             const int p = n + (int)lanes_below(sm);
@@ -162,17 +182,20 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
                 if (PREC == 0) {
                     float *zr = reinterpret_cast<float *>(zbuf) + p * (KP + 1);
 #pragma unroll
-                    for (int w = 0; w < KP; ++w) zr[w] = w < keff ? (xr[w] - mu) * rs : 0.0f;
+                    for (int w = 0; w < KP; ++w) zr[w] = w < keff ? (xv[w] - mu) * rs : 0.0f;
                 } else {
-                    unsigned short *zr = reinterpret_cast<unsigned short *>(zbuf) + p * KP;
+                    unsigned *zr = reinterpret_cast<unsigned *>(zbuf) + p * (KP / 2);
+                    unsigned hv[KP];
 #pragma unroll
                     for (int w = 0; w < KP; ++w) {
-                        const float z = w < keff ? (xr[w] - mu) * rs : 0.0f;
-                        if (PREC == 1)
-                            zr[w] = __builtin_bit_cast(unsigned short, (_Float16)z);
-                        else
-                            zr[w] = to_bf16(z);
+                        const float z = w < keff ? (xv[w] - mu) * rs : 0.0f;
+                        hv[w] = PREC == 1 ? (unsigned)__builtin_bit_cast(unsigned short, (_Float16)z)
+                                          : (unsigned)to_bf16(z);
                     }
+#pragma unroll
+                    for (int w = 0; w < KP; w += 4)
+                        *reinterpret_cast<uint2 *>(zr + w / 2) =
+                            make_uint2(hv[w] | (hv[w + 1] << 16), hv[w + 2] | (hv[w + 3] << 16));
                 }
             }
             n += __popcll(sm);
@@ -192,24 +215,45 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
 
         float S = 0.0f;  // lane li: S of lag lags[li]
         if (PREC == 0) {
+            // lags in chunks of 8: per-lane partial sums in registers, one LDS
+            // transpose-and-add per chunk gives the 8 wave totals
             const float *zb = reinterpret_cast<const float *>(zbuf);
-            for (int pc = 0; pc < n; pc += 64) {
-                const int p = pc + lane;
-                const bool act = p < n;
-                float zr[KP];
+            for (int lc = 0; lc < a.nlag; lc += 8) {
+                int mm[8];
+                float acc[8];
 #pragma unroll
-                for (int w = 0; w < KP; ++w) zr[w] = act ? zb[p * (KP + 1) + w] : 0.0f;
-                for (int li = 0; li < a.nlag; ++li) {
-                    const int m = a.lags[li];
-                    float s = 0.0f;
-                    if (act && p + m < n) {
-                        const float *zq = zb + (p + m) * (KP + 1);
-#pragma unroll
-                        for (int w = 0; w < KP; ++w) s = __builtin_fmaf(zr[w], zq[w], s);
-                    }
-                    s = wave_sum(s);
-                    if (lane == li) S += s;
+                for (int l = 0; l < 8; ++l) {
+                    mm[l] = lc + l < a.nlag ? a.lags[lc + l] : 4096;
+                    acc[l] = 0.0f;
                 }
+                for (int pc = 0; pc < n; pc += 64) {
+                    const int p = pc + lane;
+                    const bool act = p < n;
+                    float zr[KP];
+#pragma unroll
+                    for (int w = 0; w < KP; ++w) zr[w] = act ? zb[p * (KP + 1) + w] : 0.0f;
+#pragma unroll
+                    for (int l = 0; l < 8; ++l) {
+                        if (act && p + mm[l] < n) {
+                            const float *zq = zb + (p + mm[l]) * (KP + 1);
+                            float s = 0.0f;
+#pragma unroll
+                            for (int w = 0; w < KP; ++w) s = __builtin_fmaf(zr[w], zq[w], s);
+                            acc[l] += s;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int l = 0; l < 8; ++l) red[l * 65 + lane] = acc[l];
+                const int l8 = lane >> 3, part = lane & 7;
+                float s = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) s += red[l8 * 65 + part * 8 + i];
+                s += __shfl_xor(s, 1);
+                s += __shfl_xor(s, 2);
+                s += __shfl_xor(s, 4);
+                const float got = __shfl(s, ((lane - lc) & 7) * 8);
+                if (lane >= lc && lane < lc + 8 && lane < a.nlag) S = got;
             }
         } else {
             // zero rows n .. 32*nblk-1 of the last block
@@ -271,6 +315,44 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
     }
 }
 
+// Usable-element bits per output pixel: bit r of word r/64 is set when
+// mask[r, ix, z] is non-zero over the pixel's whole (clipped) window.  One
+// lane per (output pixel, element).
+__global__ __launch_bounds__(256) void k_slsc_selection(const unsigned char *__restrict__ mask,
+                                                        int nrx, int64_t nx, int64_t nz, int kwin,
+                                                        int kofs, int dsfx, int dsfz, int nxo,
+                                                        int nzo, unsigned long long *sel)
+{
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // output pixel
+    const int lane = threadIdx.x & 63;
+    if (g >= (int64_t)nxo * nzo) return;
+    const int ixo = (int)(g / nzo), izo = (int)(g - (int64_t)ixo * nzo);
+    const int64_t ix = (int64_t)ixo * dsfx, iz = (int64_t)izo * dsfz;
+    const int64_t w0 = iz + kofs < 0 ? 0 : iz + kofs;
+    const int64_t w1 = iz + kofs + kwin < nz ? iz + kofs + kwin : nz;
+    for (int c = 0; c < 4; ++c) {
+        const int r = c * 64 + lane;
+        bool ok = r < nrx;
+        if (ok) {
+            const unsigned char *m = mask + ((int64_t)r * nx + ix) * nz;
+            for (int64_t z = w0; z < w1; ++z) ok = ok && m[z] != 0;
+        }
+        const unsigned long long b = __ballot(ok);
+        if (lane == 0) sel[g * 4 + c] = b;
+    }
+}
+
+int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,
+                          int kofs, int dsfx, int dsfz, uint64_t *sel, hipStream_t s)
+{
+    const int nxo = (int)((nx + dsfx - 1) / dsfx), nzo = (int)((nz + dsfz - 1) / dsfz);
+    const int64_t npx = (int64_t)nxo * nzo;
+    hipLaunchKernelGGL(k_slsc_selection, dim3((unsigned)((npx + 3) / 4)), dim3(256), 0, s, mask,
+                       (int)nrx, nx, nz, kwin, kofs, dsfx, dsfz, nxo, nzo,
+                       reinterpret_cast<unsigned long long *>(sel));
+    return check_launch("slsc_selection");
+}
+
 // Staging geometry and LDS size for a launch; returns 0 or an error status.
 static int slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
 {
@@ -278,21 +360,23 @@ static int slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
     const int npad = (nrx + 31) / 32 * 32;
     a.zbytes = (int)align_up(prec == 0 ? (size_t)npad * (kp + 1) * 4 : (size_t)npad * kp * 2, 16);
     const size_t budget = 64 * 1024;
-    // strip length: keep the staged channels within about 32 KB
+    a.wbytes = (int)align_up((size_t)a.zbytes + 256 + (prec == 0 ? 8 * 65 * 4 : 0), 16);
+    // strip length: keep the staged channels within about 32 KB (PYCBF_SLSC_PZ
+    // overrides it, still bounded by the LDS budget below)
     int pz = 16;
     for (; pz > 1; --pz) {
         const int span = (pz - 1) * a.dsfz + a.kwin;
         if ((size_t)nrx * (span + 1) * 4 <= 32 * 1024) break;
     }
+    if (const char *e = getenv("PYCBF_SLSC_PZ")) pz = std::max(1, std::min(64, atoi(e)));
     pz = std::min<int>(pz, a.nzo);
     for (;;) {
         a.pz = pz;
         a.span = (pz - 1) * a.dsfz + a.kwin;
         a.xpitch = a.span | 1;
-        const size_t stage = align_up((size_t)nrx * a.xpitch * 4 + (a.apod ? (size_t)nrx * a.span : 0),
-                                      16);
+        const size_t stage = align_up((size_t)nrx * a.xpitch * 4, 16);
         for (nwave = std::min(4, pz); nwave >= 1; --nwave) {
-            lds = stage + (size_t)nwave * a.zbytes + (size_t)nwave * 256;
+            lds = stage + (size_t)nwave * a.wbytes;
             if (lds <= budget) return 0;
         }
         if (pz == 1) break;
@@ -323,12 +407,12 @@ static int launch_slsc_p(const SlscArgs &a, int kp, int nwave, size_t lds, int64
 }
 
 int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz,
-                const float *apod, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
+                const uint64_t *sel, int kwin, int kofs, int dsfx, int dsfz, const int32_t *lags,
                 int nlag, int precision, float *out, hipStream_t s)
 {
     SlscArgs a{};
     a.im = im;
-    a.apod = apod;
+    a.sel = reinterpret_cast<const unsigned long long *>(sel);
     a.out = out;
     a.nrx = nrx;
     a.nx = nx;

@@ -80,6 +80,8 @@ _SIGS = {
                    ct.c_int),
     "pycbf_slsc": ([_vp, _i64, _i64, _i64, _i64, _vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                     _vp, ct.c_int, ct.c_int, _vp, _vp], ct.c_int),
+    "pycbf_slsc_selection": ([_vp, _i64, _i64, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                              _vp, _vp], ct.c_int),
     "pycbf_profile_enable": ([ct.c_int], None),
     "pycbf_profile_reset": ([], None),
     "pycbf_profile_read": ([ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)],

@@ -75,11 +75,43 @@ class SyntheticDMASBeamformer(SyntheticBeamformer, DMASBeamformer, _GPUDMAS):
         return self._dmas_call(SyntheticBeamformer.__call__, txrxt, out_as_numpy, buffer)
 
 
-def slsc(im, lags, kwin, kofs=None, apod=None, dsfx=1, dsfz=1, precision="fp16", out=None):
+def usable_mask(apod, device):
+    """uint8 mask (non-zero apodisation) on ``device``; uint8 tensors pass through."""
+    torch = _torch()
+    if isinstance(apod, torch.Tensor) and apod.dtype == torch.uint8:
+        return apod.to(device).contiguous()
+    a = apod if isinstance(apod, torch.Tensor) else torch.as_tensor(np.asarray(apod))
+    return (a.to(device) != 0).to(torch.uint8).contiguous()
+
+
+def slsc_selection(apod, kwin, kofs=None, dsfx=1, dsfz=1, device=None):
+    """Usable-element bits per SLSC output pixel (pycbf_slsc_selection): an
+    (nxo, nzo, 4) int64 CUDA tensor from the (nrx, nx, nz) receive
+    apodisation (non-zero = usable)."""
+    torch = _torch()
+    if device is None:
+        device = apod.device if isinstance(apod, torch.Tensor) and apod.is_cuda else "cuda"
+    mask = usable_mask(apod, device)
+    if mask.dim() != 3:
+        raise BeamformerException("apod must be (nrx, nx, nz)")
+    nrx, nx, nz = mask.shape
+    kofs = -(int(kwin) // 2) if kofs is None else int(kofs)
+    sel = torch.empty(((nx + dsfx - 1) // dsfx, (nz + dsfz - 1) // dsfz, 4), dtype=torch.int64,
+                      device=mask.device)
+    rc = E.lib.pycbf_slsc_selection(E.ptr(mask), nrx, nx, nz, int(kwin), kofs, int(dsfx),
+                                    int(dsfz), E.ptr(sel), E.stream_of(mask.device))
+    E.check(rc, "pycbf_slsc_selection")
+    return sel
+
+
+def slsc(im, lags, kwin, kofs=None, apod=None, dsfx=1, dsfz=1, precision="fp16", out=None,
+         sel=None):
     """SLSC of per-receive-channel images on the GPU (pycbf_slsc).
 
-    im: float32 CUDA tensor (nrx, nx, nz) or (nfr, nrx, nx, nz); apod: None or
-    (nrx, nx, nz); returns (nfr?, ceil(nx/dsfx), ceil(nz/dsfz)) float32."""
+    im: float32 CUDA tensor (nrx, nx, nz) or (nfr, nrx, nx, nz).  Elements are
+    selected by ``sel`` (from slsc_selection, reusable across frames) or, when
+    it is None, by ``apod`` ((nrx, nx, nz), non-zero = usable; None = all).
+    Returns (nfr?, ceil(nx/dsfx), ceil(nz/dsfz)) float32."""
     torch = _torch()
     if not (isinstance(im, torch.Tensor) and im.is_cuda and im.dtype == torch.float32):
         raise BeamformerException("im must be a float32 CUDA tensor")
@@ -90,16 +122,20 @@ def slsc(im, lags, kwin, kofs=None, apod=None, dsfx=1, dsfz=1, precision="fp16",
     nfr, nrx, nx, nz = im4.shape
     kofs = -(int(kwin) // 2) if kofs is None else int(kofs)
     lags_h = np.ascontiguousarray(np.asarray(lags).ravel(), np.int32)
-    if apod is not None:
-        apod = apod.to(device=im.device, dtype=torch.float32).contiguous()
-        if tuple(apod.shape) != (nrx, nx, nz):
+    oshape = ((nx + dsfx - 1) // dsfx, (nz + dsfz - 1) // dsfz)
+    if sel is None and apod is not None:
+        if tuple(np.shape(apod)) != (nrx, nx, nz):
             raise BeamformerException(f"apod must be {(nrx, nx, nz)}")
-    shape = (nfr, (nx + dsfx - 1) // dsfx, (nz + dsfz - 1) // dsfz)
+        sel = slsc_selection(apod, kwin, kofs, dsfx, dsfz, device=im.device)
+    if sel is not None and (tuple(sel.shape) != oshape + (4,) or sel.dtype != torch.int64
+                            or not sel.is_contiguous()):
+        raise BeamformerException(f"sel must be a contiguous int64 tensor of shape {oshape + (4,)}")
+    shape = (nfr,) + oshape
     if out is None:
         out = torch.empty(shape, dtype=torch.float32, device=im.device)
     elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
         raise BeamformerException(f"out must be a contiguous float32 tensor of shape {shape}")
-    rc = E.lib.pycbf_slsc(E.ptr(im4), nfr, nrx, nx, nz, E.ptr(apod), int(kwin), kofs, int(dsfx),
+    rc = E.lib.pycbf_slsc(E.ptr(im4), nfr, nrx, nx, nz, E.ptr(sel), int(kwin), kofs, int(dsfx),
                           int(dsfz), lags_h.ctypes.data, lags_h.size,
                           E.SLSC_PRECISION[precision], E.ptr(out), E.stream_of(im.device))
     E.check(rc, "pycbf_slsc")
@@ -112,7 +148,8 @@ class _GPUSLSC:
         if nx * nz != self.nop:
             raise BeamformerException(
                 f"gridshape {self.gridshape} holds {nx * nz} pixels but there are {self.nop}")
-        self._slsc_apod = apodrx.reshape(self.nrx, nx, nz).contiguous()
+        self._slsc_sel = slsc_selection(apodrx.reshape(self.nrx, nx, nz), self.kwin, self.kofs,
+                                        self.dsfx, self.dsfz, device=self._dev)
 
     def _slsc_call(self, base_call, txrxt, out_as_numpy, buffer):
         torch = _torch()
@@ -130,9 +167,9 @@ class _GPUSLSC:
                     and out.is_contiguous()):
                 raise BeamformerException(
                     f"buffer must be a contiguous float32 tensor of shape {shape} on {self._dev}")
-        slsc(imrx.reshape(nf, self.nrx, nx, nz), self.lags, self.kwin, self.kofs,
-             self._slsc_apod, self.dsfx, self.dsfz, self.precision,
-             out=out.reshape((nf,) + self._slsc_outshape()))
+        slsc(imrx.reshape(nf, self.nrx, nx, nz), self.lags, self.kwin, self.kofs, None,
+             self.dsfx, self.dsfz, self.precision, out=out.reshape((nf,) + self._slsc_outshape()),
+             sel=self._slsc_sel)
         return out.cpu().numpy() if out_as_numpy else out
 
 

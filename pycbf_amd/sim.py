@@ -88,6 +88,16 @@ def scatterers(n_points, n_speckle, xlim, zlim, seed):
     return xs, zs, amp
 
 
+def cyst_speckle(n_speckle, xlim, zlim, center, radius, seed):
+    """Diffuse speckle (normal amplitudes) with an anechoic disc removed."""
+    rng = np.random.default_rng(seed)
+    xs = rng.uniform(xlim[0], xlim[1], n_speckle)
+    zs = rng.uniform(zlim[0], zlim[1], n_speckle)
+    amp = rng.standard_normal(n_speckle)
+    keep = (xs - center[0]) ** 2 + (zs - center[1]) ** 2 > radius ** 2
+    return xs[keep], zs[keep], amp[keep]
+
+
 def two_way_delays(xs, zs, steers, xele, c=C0):
     """(ntx, nrx, ns) plane-wave transmit + receive delays."""
     steers = np.asarray(steers, np.float64)
@@ -175,3 +185,18 @@ def workload_tabbed_small(seed=0, ntx=5, nrx=128, nt=2048, nx=128, nz=256, n_poi
     tabs = pw_tabs(steers, xele, pnts)
     return dict(rf=rf, steers=steers, xele=xele, pnts=pnts, xout=xout, zout=zout, t0=0.0,
                 dt=1.0 / FS, nt=nt, tabs=tabs)
+
+
+def workload_slsc(nrx=128, ntx=75, nx=256, nz=256, sub=8, xlim=(-19e-3, 19e-3),
+                  zlim=(2e-3, 42e-3)):
+    """Coherence workload geometry: 75 plane waves over +-16 deg, 128 elements,
+    an output grid nx x nz over a 40 mm depth whose axial pixels each span
+    ``sub`` fine pixels of c/(2 fs) (one RF sample of two-way time), so an
+    8-sample axial kernel is one wavelength at 5 MHz and 40 MHz sampling.
+    Returns steers, xele, the synthetic parameters on the fine grid
+    (nx * nz*sub pixels, x-major) and the fine grid shape."""
+    xele = linear_array(nrx)
+    steers = np.radians(np.linspace(-16, 16, ntx))
+    nzf = nz * sub
+    _, _, pnts = grid(nx, nzf, xlim, zlim)
+    return steers, xele, pw_synthetic(steers, xele, pnts), (nx, nzf)

@@ -142,3 +142,24 @@ def test_synthetic_slsc_beamformer_frames():
         ref = numerics.slsc_image(imrx.reshape(-1, nx, nz), tabs[3].reshape(-1, nx, nz),
                                   np.arange(1, 6), 3, -1)
         _compare(got[f], ref, TOL["fp32"], f"synthetic SLSC frame {f}")
+
+
+def test_slsc_selection_bits_match_window_rule():
+    """pycbf_slsc_selection: bit r set iff the mask is non-zero over the whole
+    clipped window of the output pixel."""
+    import torch
+    from pycbf_amd.gpu import slsc_selection
+    rng = np.random.default_rng(21)
+    nrx, nx, nz, kwin, kofs, dsfx, dsfz = 200, 5, 37, 6, -3, 2, 4
+    mask = (rng.uniform(size=(nrx, nx, nz)) > 0.02).astype(np.uint8)
+    sel = slsc_selection(torch.as_tensor(mask, device="cuda"), kwin, kofs, dsfx, dsfz)
+    bits = sel.cpu().numpy().view(np.uint64)
+    for ixo in range(bits.shape[0]):
+        for izo in range(bits.shape[1]):
+            ix, iz = ixo * dsfx, izo * dsfz
+            z0, z1 = max(0, iz + kofs), min(nz, iz + kofs + kwin)
+            ok = np.all(mask[:, ix, z0:z1] != 0, axis=1)
+            exp = np.zeros(4, np.uint64)
+            for r in np.nonzero(ok)[0]:
+                exp[r // 64] |= np.uint64(1) << np.uint64(r % 64)
+            assert np.array_equal(bits[ixo, izo], exp), (ixo, izo)
