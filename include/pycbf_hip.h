@@ -170,6 +170,28 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                int nlag, int dmastype, int sumlags, float *out, void *stream);
 
+/* Bytes of workspace pycbf_das_tabbed_bwd needs (cubic: spline table and
+ * cotangent bins of every channel; all kinds: per-tile delay ranges). */
+size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
+                                     int interp_kind);
+
+/* Adjoint of pycbf_das_tabbed (no reference counterpart: pycbf is forward
+ * only; the forward it differentiates is pycbf/cpu/pycbf.c:179-271 with the
+ * layouts of pycbf/gpu/__engines__.cu:813-864).  Given gout, the gradient of
+ * a loss with respect to out (layout by sumflag as pycbf_das_tabbed), writes
+ *   grf      (nfr, ntx, nrx, nt)  d loss / d rf
+ *   gtautx, gapodtx (ntx, np); gtaurx, gapodrx (nrx, np)
+ * Any output pointer may be NULL (not computed).  interp_kind nearest (usf 1;
+ * its delay gradient is 0), linear or cubic (natural spline: the adjoint of
+ * the spline prefilter is applied to grf).  Samples with |apodtx*apodrx| <
+ * thresh contribute nothing, as in the forward. */
+int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                         const float *tautx, const float *apodtx, const float *taurx,
+                         const float *apodrx, int64_t np, int interp_kind, float thresh,
+                         int sumflag, const float *gout, float *grf, float *gtautx,
+                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
 /* Short-lag spatial coherence.  Replaces calc_lagn_coherence and the pixel loop
  * of examples/advanced_beamforming.ipynb:65-72,205-223 (the reference has no
  * GPU SLSC).

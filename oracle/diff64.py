@@ -1,0 +1,103 @@
+"""FP64 differentiable restatement of the Tabbed DAS forward, for checking the
+GPU adjoint kernels (test infrastructure only: imported by tests/, never by
+the product).
+
+Forward model (pycbf/cpu/pycbf.c:68-105,179-271 for nearest and cubic; the
+project-defined linear kind), written with torch float64 tensors so that
+torch.autograd gives reference gradients with respect to the channel data,
+delays and apodisations:
+
+  x = tautx[tx, p] + taurx[rx, p]  (summed in float32 as the kernels do, or in float64)
+  w = apodtx[tx, p] * apodrx[rx, p],  used where |w| >= thresh
+  j = floor((x - t0) / dt) clamped to n-1, zero outside [t0, t0 + dt (n-1)]
+  a = (t0 + dt (j+1) - x) / dt,  b = (x - (t0 + dt j)) / dt
+  linear: v = a y_j + b y_j+1;  cubic: v += dt^2/6 ((a^3-a) y2_j + (b^3-b) y2_j+1)
+  nearest: v = y_k, k = (int)((x - t0)/dt + 0.5), zero unless -1 < (x - t0)/dt + 0.5 < n
+with y_n = y2_n = 0 (the zero entry past the trace) and y2 the natural
+spline's second derivatives (y2_0 = y2_{n-1} = 0), solved exactly in float64.
+
+Pinned by tests/test_diff_reference.py: central finite differences of this
+forward, linearity in the channel data, and agreement with the float32
+oracle (oracle.das_tabbed) on the forward values.
+"""
+from __future__ import annotations
+
+import torch
+
+
+def natural_y2(y: torch.Tensor, dt: float) -> torch.Tensor:
+    """Second derivatives of the natural cubic spline through y (..., n)."""
+    n = y.shape[-1]
+    if n < 3:
+        return torch.zeros_like(y)
+    m = n - 2
+    T = (torch.diag(torch.full((m,), 4.0, dtype=y.dtype, device=y.device))
+         + torch.diag(torch.ones(m - 1, dtype=y.dtype, device=y.device), 1)
+         + torch.diag(torch.ones(m - 1, dtype=y.dtype, device=y.device), -1))
+    rhs = (y[..., 2:] - 2.0 * y[..., 1:-1] + y[..., :-2]) * (6.0 / (dt * dt))
+    flat = rhs.reshape(-1, m).T
+    sol = torch.linalg.solve(T, flat).T.reshape(rhs.shape)
+    z = torch.zeros(y.shape[:-1] + (1,), dtype=y.dtype, device=y.device)
+    return torch.cat([z, sol, z], dim=-1)
+
+
+def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="tx_and_rx",
+               thresh=0.0, x32=True):
+    """rf (nfr, ntx, nrx, n); tables (ntx, np) / (nrx, np); returns float64
+    with the sumtype layout of the GPU beamformers (frames first).  x32 sums
+    the delays in float32 as the kernels do (same sample indices); x32=False
+    keeps the whole function in float64 (finite-difference checks)."""
+    rf = rf.to(torch.float64)
+    nfr, ntx, nrx, n = rf.shape
+    if x32:
+        # float32 delay sum, and sample indices / trace bounds from float32
+        # arithmetic on the float32 time axis, as the kernels select them
+        xs = tautx.to(torch.float32)[:, None, :] + taurx.to(torch.float32)[None, :, :]
+        x = xs.to(torch.float64)
+        t0f = torch.tensor(t0, dtype=torch.float32)
+        dtf = torch.tensor(dt, dtype=torch.float32)
+        q = ((xs.detach() - t0f) / dtf)
+        tmaxf = t0f + dtf * torch.tensor(float(n - 1), dtype=torch.float32)
+        t0, dt = float(t0f), float(dtf)
+        lo_ok = xs.detach() >= t0f
+        hi_ok = xs.detach() <= tmaxf
+        qd = q.to(torch.float64)
+    else:
+        x = tautx.to(torch.float64)[:, None, :] + taurx.to(torch.float64)[None, :, :]
+        qd = (x.detach() - t0) / dt
+        tmax = t0 + dt * (n - 1)
+        lo_ok = x.detach() >= t0
+        hi_ok = x.detach() <= tmax
+    w = apodtx.to(torch.float64)[:, None, :] * apodrx.to(torch.float64)[None, :, :]
+    keep = (w.abs() >= thresh).to(torch.float64)
+    zpad = torch.zeros(rf.shape[:-1] + (1,), dtype=rf.dtype, device=rf.device)
+    y = torch.cat([rf, zpad], dim=-1)                         # (nfr, ntx, nrx, n+1)
+
+    def take(arr, idx):
+        idx = idx.clamp(0, n).expand(nfr, *idx.shape)
+        return torch.gather(arr, -1, idx)
+
+    if kind == "nearest":
+        # (int)(q + 0.5) with the 0.5 added in double: q in (-1.5, -0.5) selects
+        # sample 0 (pycbf/cpu/pycbf.c:264,267)
+        d = qd + 0.5
+        inside = ((d > -1.0) & (d < n)).to(torch.float64)
+        v = take(y, torch.trunc(d).clamp(0, n - 1).long())
+    else:
+        inside = (lo_ok & hi_ok).to(torch.float64)
+        j = torch.floor(qd).clamp(0, n - 1).long()
+        jf = j.to(torch.float64)
+        a = (t0 + dt * (jf + 1) - x) / dt
+        b = (x - (t0 + dt * jf)) / dt
+        v = a * take(y, j) + b * take(y, j + 1)
+        if kind == "cubic":
+            y2 = torch.cat([natural_y2(rf, dt), zpad], dim=-1)
+            v = v + (dt * dt / 6.0) * ((a ** 3 - a) * take(y2, j) + (b ** 3 - b) * take(y2, j + 1))
+    s = v * (w * keep * inside)[None]                        # (nfr, ntx, nrx, np)
+    if sumtype == "none":
+        return s
+    if sumtype == "tx_only":
+        return s.sum(dim=1)
+    if sumtype == "rx_only":
+        return s.sum(dim=2)
+    return s.sum(dim=(1, 2))

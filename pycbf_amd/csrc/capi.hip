@@ -479,6 +479,109 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
                                 info->tinfo.x0, dt, n, idx, (hipStream_t)stream);
 }
 
+// ------------------------------------------------------------- backward ---
+struct BwdPlan {
+    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, total = 0;
+};
+
+static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind)
+{
+    BwdPlan b;
+    const int64_t nt = info->tinfo.nx;
+    const int64_t nch = nfr * info->ntx * info->nrx;
+    b.bounds = align_up((size_t)das_tiles(np) * (size_t)(info->ntx + info->nrx) * 2 * 4, 256);
+    if (kind == PYCBF_INTERP_CUBIC) {
+        b.coef = align_up((size_t)nch * (size_t)(nt + 1) * 8, 256);
+        b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
+        b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
+    }
+    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + 5 * 256;
+    return b;
+}
+
+size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
+                                     int interp_kind)
+{
+    if (!info || nfr < 1 || np < 1) return 0;
+    return plan_bwd(info, nfr, np, interp_kind).total;
+}
+
+int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                         const float *tautx, const float *apodtx, const float *taurx,
+                         const float *apodrx, int64_t np, int interp_kind, float thresh,
+                         int sumflag, const float *gout, float *grf, float *gtautx,
+                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
+                         size_t workspace_bytes, void *stream)
+{
+    if (interp_kind != PYCBF_INTERP_NEAREST && interp_kind != PYCBF_INTERP_LINEAR &&
+        interp_kind != PYCBF_INTERP_CUBIC)
+        return fail(PYCBF_EINVAL, "the adjoint supports nearest, linear and cubic (got %d)",
+                    interp_kind);
+    int rc = validate_common(info, rf, nfr, np, interp_kind, 1, nullptr, sumflag, gout);
+    if (rc) return rc;
+    if (!tautx || !apodtx || !taurx || !apodrx)
+        return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
+    if (nfr > 65535 || info->ntx > 65535) return fail(PYCBF_EINVAL, "nfr and ntx must be <= 65535");
+    const BwdPlan plan = plan_bwd(info, nfr, np, interp_kind);
+    if (!workspace || workspace_bytes < plan.total)
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
+                    workspace_bytes);
+    hipStream_t s = (hipStream_t)stream;
+    Carver cv(workspace, workspace_bytes);
+    const int nt = info->tinfo.nx;
+    const float t0 = info->tinfo.x0, dt = info->tinfo.dx;
+    const int64_t nch = nfr * info->ntx * info->nrx;
+
+    BwdArgs b{};
+    b.ttx = tautx;
+    b.atx = apodtx;
+    b.trx = taurx;
+    b.arx = apodrx;
+    b.np = np;
+    b.ntx = (int)info->ntx;
+    b.nrx = (int)info->nrx;
+    b.nfr = (int)nfr;
+    b.nt = nt;
+    b.src = rf;
+    b.t0 = t0;
+    b.dt = dt;
+    b.rdt = 1.0f / dt;
+    b.tmax = t0 + dt * (float)(nt - 1);
+    b.thresh = thresh;
+    b.sumflag = sumflag;
+    b.gout = gout;
+    float *bounds = (float *)cv.take(plan.bounds);
+    b.bounds = bounds;
+    if (interp_kind == PYCBF_INTERP_CUBIC) {
+        float2 *coef = (float2 *)cv.take(plan.coef);
+        float *u = (float *)cv.take(plan.uscr);
+        b.gy = (float *)cv.take(plan.gy);
+        b.ge = (float *)cv.take(plan.ge);
+        b.gstride = nt + 1;
+        if ((rc = launch_spline_coefs(rf, nch, nt, dt, u, coef, s))) return rc;
+        b.src = (const float *)coef;
+    } else {
+        b.gy = grf;
+        b.gstride = nt;
+    }
+    if (grf) {
+        DasArgs a{};
+        a.ttx = tautx;
+        a.atx = apodtx;
+        a.trx = taurx;
+        a.arx = apodrx;
+        a.np = np;
+        a.ntx = b.ntx;
+        a.nrx = b.nrx;
+        a.thresh = thresh;
+        a.geo = 0;
+        if ((rc = launch_tile_bounds(a, bounds, s))) return rc;
+    }
+    const int gk = interp_kind == PYCBF_INTERP_NEAREST ? GK_NEAREST
+                   : (interp_kind == PYCBF_INTERP_LINEAR ? GK_LINEAR : GK_CUBIC);
+    return launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+}
+
 int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                int nlag, int dmastype, int sumlags, float *out, void *stream)
 {

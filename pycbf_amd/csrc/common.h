@@ -106,6 +106,24 @@ int launch_synth_tables(int ndim, int ntx, int nrx, const float *txpack, const f
 int launch_debug_indices(int ntx, int nrx, const float *ttx, const float *trx, int64_t np,
                          int nearest, float t0, float dt, int n, int32_t *idx, hipStream_t s);
 
+// -------------------------------------------------------------- backward ---
+// Arguments of the Tabbed DAS adjoint kernels (das_bwd.hip).
+struct BwdArgs {
+    const float *ttx, *atx, *trx, *arx;  // (ntx, np), (nrx, np)
+    int64_t np;
+    int ntx, nrx, nfr, nt;
+    const float *src;  // raw rf (nearest, linear) or the (y, e) table (cubic)
+    float t0, dt, rdt, tmax, thresh;
+    int sumflag;
+    const float *gout;
+    const float *bounds;  // (ntiles, ntx + nrx, 2) delay ranges of contributing pixels
+    float *gtau, *gapod;  // outer-axis gradients (may be null)
+    float *gy, *ge;       // channel-data cotangent bins (ge: cubic only)
+    int gstride;          // floats per channel row of gy / ge
+};
+int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
+                   float *gapodrx, float *grf, hipStream_t s);
+
 // ------------------------------------------------------------------ DMAS ---
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                 int nlag, int dmastype, int sumlags, float *out, hipStream_t s);

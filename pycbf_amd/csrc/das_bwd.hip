@@ -1,0 +1,355 @@
+// Adjoint (backward) kernels of the Tabbed DAS, for gradients of the images
+// with respect to the channel data, the delays and the apodisations.
+//
+// Forward model (pycbf/cpu/pycbf.c:179-207 with the GPU layouts of
+// pycbf/gpu/__engines__.cu:813-864):
+//   out[f, (tx), (rx), p] += w * v(rf[f, tx, rx, :], x),
+//   w = apodtx[tx, p] * apodrx[rx, p] (only where |w| >= thresh),
+//   x = tautx[tx, p] + taurx[rx, p],
+// with v the nearest / linear / natural-cubic interpolant of the forward
+// kernels (same knot index, same separately rounded weights a and b, zero
+// outside [t0, t0 + dt (nt-1)]).
+//
+// Gradients, for an upstream gradient G of the output:
+//   d tautx[tx, p]  = sum_f sum_rx G w v'(x)     d apodtx[tx, p] = sum_f sum_rx G apodrx v(x)
+//   d taurx[rx, p]  = sum_f sum_tx G w v'(x)     d apodrx[rx, p] = sum_f sum_tx G apodtx v(x)
+//   d rf[f, tx, rx, k] = sum_p G w dv(x)/dy_k                       (transposed gather)
+// k_das_bwd_tau evaluates the first four with one lane per (element, pixel),
+// recomputing x and w from the tables.  k_das_bwd_rf scatters the
+// channel-data gradient: one workgroup per (pixel tile, transmit, frame)
+// walks the receive channels, accumulates the tile's contributions in LDS
+// bins that cover the tile's sample window (k_tile_bounds), and flushes the
+// bins to HBM with one float atomic per non-zero bin.  For the natural
+// cubic the bins hold the cotangents of the samples y and of the table's
+// e = dt^2/6 y2, and k_spline_adjoint adds the transposed spline solve
+//   dy += D2(T^-1 de),  T = tridiag(1, 4, 1),  D2 the second difference,
+// which is the adjoint of e = T^-1 D2 y (the natural spline with y2 = 0 at
+// both ends).
+#include <math.h>
+
+#include "common.h"
+#include "das_dev.h"
+#include "prof.h"
+
+namespace pycbf {
+
+namespace {
+
+constexpr int BWD_BINS = 2048;  // LDS bins per array and channel window
+
+// value and derivative of the interpolant at x (0 outside the trace)
+template <int GK>
+__device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t row, int n,
+                                          float x, float t0, float dt, float rdt, float tmax,
+                                          float &v, float &d)
+{
+    if (GK == GK_NEAREST) {
+        int it = 0;
+        if (!nearest_index(x, t0, dt, n, it)) return false;
+        v = src[row * n + it];
+        d = 0.0f;
+        return true;
+    } else {
+        int j = 0;
+        if (!knot_index(x, t0, dt, tmax, n, j)) return false;
+        const float xlead = t0 + dt * (float)(j + 1);
+        const float xlag = t0 + dt * (float)j;
+        if (GK == GK_LINEAR) {
+            const float a = (xlead - x) / dt;
+            const float b = (x - xlag) / dt;
+            const float y0 = src[row * n + j];
+            const float y1 = (j + 1 < n) ? src[row * n + j + 1] : 0.0f;
+            v = a * y0 + b * y1;
+            d = (y1 - y0) / dt;
+        } else {
+            const float a = (xlead - x) * rdt;
+            const float b = (x - xlag) * rdt;
+            const float ca = __builtin_fmaf(a * a, a, -a);
+            const float cb = __builtin_fmaf(b * b, b, -b);
+            const Pair2 c = *reinterpret_cast<const Pair2 *>(src + (row * (n + 1) + j) * 2);
+            float r = a * c.y0;
+            r = __builtin_fmaf(b, c.y1, r);
+            r = __builtin_fmaf(ca, c.z0, r);
+            v = __builtin_fmaf(cb, c.z1, r);
+            const float da = __builtin_fmaf(3.0f * a, a, -1.0f);
+            const float db = __builtin_fmaf(3.0f * b, b, -1.0f);
+            d = ((c.y1 - c.y0) + (db * c.z1 - da * c.z0)) * rdt;
+        }
+        return true;
+    }
+}
+
+__device__ __forceinline__ int64_t gout_index(int sumflag, int f, int tx, int rx, int ntx, int nrx,
+                                              int64_t np, int64_t p)
+{
+    switch (sumflag) {
+    case PYCBF_SUM_NONE: return (((int64_t)f * ntx + tx) * nrx + rx) * np + p;
+    case PYCBF_SUM_TX_ONLY: return ((int64_t)f * nrx + rx) * np + p;
+    case PYCBF_SUM_RX_ONLY: return ((int64_t)f * ntx + tx) * np + p;
+    default: return (int64_t)f * np + p;
+    }
+}
+
+}  // namespace
+
+// one lane per (outer element, pixel); outer axis tx (OUTER_TX) or rx
+template <int GK, int OUTER_TX>
+__global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
+{
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int o = blockIdx.y;
+    if (p >= A.np) return;
+    const int ninner = OUTER_TX ? A.nrx : A.ntx;
+    const float *to = OUTER_TX ? A.ttx : A.trx;
+    const float *ao = OUTER_TX ? A.atx : A.arx;
+    const float *ti = OUTER_TX ? A.trx : A.ttx;
+    const float *ai = OUTER_TX ? A.arx : A.atx;
+    const float tau_o = to[(int64_t)o * A.np + p];
+    const float ap_o = ao[(int64_t)o * A.np + p];
+    float gt = 0.0f, ga = 0.0f;
+    for (int i = 0; i < ninner; ++i) {
+        const float ap_i = ai[(int64_t)i * A.np + p];
+        const float w = OUTER_TX ? ap_o * ap_i : ap_i * ap_o;
+        if (!(fabsf(w) >= A.thresh)) continue;
+        const float x = OUTER_TX ? tau_o + ti[(int64_t)i * A.np + p] : ti[(int64_t)i * A.np + p] + tau_o;
+        const int tx = OUTER_TX ? o : i, rx = OUTER_TX ? i : o;
+        for (int f = 0; f < A.nfr; ++f) {
+            const int64_t row = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
+            float v = 0.0f, d = 0.0f;
+            if (!interp_vd<GK>(A.src, row, A.nt, x, A.t0, A.dt, A.rdt, A.tmax, v, d)) break;
+            const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)];
+            gt = __builtin_fmaf(g * w, d, gt);
+            ga = __builtin_fmaf(g * ap_i, v, ga);
+        }
+    }
+    if (A.gtau) A.gtau[(int64_t)o * A.np + p] = gt;
+    if (A.gapod) A.gapod[(int64_t)o * A.np + p] = ga;
+}
+
+// transposed gather: one workgroup per (pixel tile, transmit, frame)
+template <int GK>
+__global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
+{
+    __shared__ float by[BWD_BINS];
+    __shared__ float be[GK == GK_CUBIC ? BWD_BINS : 1];
+    const int64_t tile = blockIdx.x;
+    const int tx = blockIdx.y, f = blockIdx.z;
+    const int64_t p = tile * 256 + threadIdx.x;
+    const bool act = p < A.np;
+    const int nent = A.ntx + A.nrx;
+    const float *bt = A.bounds + (tile * nent + tx) * 2;
+    const float txlo = bt[0], txhi = bt[1];
+    const float ttx = act ? A.ttx[(int64_t)tx * A.np + p] : 0.0f;
+    const float atx = act ? A.atx[(int64_t)tx * A.np + p] : 0.0f;
+    for (int t = threadIdx.x; t < BWD_BINS; t += 256) {
+        by[t] = 0.0f;
+        if (GK == GK_CUBIC) be[t] = 0.0f;
+    }
+    __syncthreads();
+    for (int rx = 0; rx < A.nrx; ++rx) {
+        const float *br = A.bounds + (tile * nent + A.ntx + rx) * 2;
+        const float xlo = txlo + br[0], xhi = txhi + br[1];
+        int lo = 0, cnt = 0;
+        if (xlo <= xhi) {
+            if (GK == GK_NEAREST) {
+                const double dlo = (double)((xlo - A.t0) / A.dt) + 0.5;
+                const double dhi = (double)((xhi - A.t0) / A.dt) + 0.5;
+                if (dhi > -1.0 && dlo < (double)A.nt) {
+                    lo = (dlo > -1.0) ? (int)dlo : 0;
+                    const int hi = (dhi < (double)A.nt) ? (int)dhi : A.nt - 1;
+                    cnt = hi - lo + 1;
+                }
+            } else if (!(xhi < A.t0 || xlo > A.tmax)) {
+                lo = (xlo < A.t0) ? 0 : min((int)((xlo - A.t0) / A.dt), A.nt - 1);
+                const int hi = (xhi > A.tmax) ? A.nt - 1 : min((int)((xhi - A.t0) / A.dt), A.nt - 1);
+                cnt = hi - lo + 2;
+            }
+        }
+        if (cnt == 0) continue;  // uniform: no pixel of the tile uses this channel
+        const bool binned = cnt <= BWD_BINS;
+        const int64_t chan = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
+        float *gy = A.gy + chan * A.gstride;
+        float *ge = (GK == GK_CUBIC) ? A.ge + chan * A.gstride : nullptr;
+        if (act) {
+            const float arx = A.arx[(int64_t)rx * A.np + p];
+            const float w = atx * arx;
+            if (fabsf(w) >= A.thresh) {
+                const float x = ttx + A.trx[(int64_t)rx * A.np + p];
+                const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)] * w;
+                if (GK == GK_NEAREST) {
+                    int it = 0;
+                    if (nearest_index(x, A.t0, A.dt, A.nt, it) && g != 0.0f) {
+                        if (binned) atomicAdd(&by[it - lo], g);
+                        else unsafeAtomicAdd(gy + it, g);
+                    }
+                } else {
+                    int j = 0;
+                    if (knot_index(x, A.t0, A.dt, A.tmax, A.nt, j) && g != 0.0f) {
+                        const float xlead = A.t0 + A.dt * (float)(j + 1);
+                        const float xlag = A.t0 + A.dt * (float)j;
+                        float a, b;
+                        if (GK == GK_LINEAR) {
+                            a = (xlead - x) / A.dt;
+                            b = (x - xlag) / A.dt;
+                        } else {
+                            a = (xlead - x) * A.rdt;
+                            b = (x - xlag) * A.rdt;
+                        }
+                        // sample j+1 exists for the cubic table (zero entry at nt); for
+                        // linear the forward reads 0 past the last sample
+                        const bool up = (GK == GK_CUBIC) || (j + 1 < A.nt);
+                        if (binned) {
+                            atomicAdd(&by[j - lo], g * a);
+                            if (up) atomicAdd(&by[j + 1 - lo], g * b);
+                        } else {
+                            unsafeAtomicAdd(gy + j, g * a);
+                            if (up) unsafeAtomicAdd(gy + j + 1, g * b);
+                        }
+                        if (GK == GK_CUBIC) {
+                            const float ca = __builtin_fmaf(a * a, a, -a);
+                            const float cb = __builtin_fmaf(b * b, b, -b);
+                            if (binned) {
+                                atomicAdd(&be[j - lo], g * ca);
+                                atomicAdd(&be[j + 1 - lo], g * cb);
+                            } else {
+                                unsafeAtomicAdd(ge + j, g * ca);
+                                unsafeAtomicAdd(ge + j + 1, g * cb);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (!binned) continue;  // uniform
+        __syncthreads();
+        // flush the window and clear the bins for the next channel
+        for (int t = threadIdx.x; t < cnt; t += 256) {
+            const float vy = by[t];
+            by[t] = 0.0f;
+            if (vy != 0.0f && lo + t < A.gstride) unsafeAtomicAdd(gy + lo + t, vy);
+            if (GK == GK_CUBIC) {
+                const float ve = be[t];
+                be[t] = 0.0f;
+                if (ve != 0.0f) unsafeAtomicAdd(ge + lo + t, ve);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// drf[k] = gy[k] + (u[k-1] - 2u[k] + u[k+1]) with T u = ge[1..n-2], u = 0
+// outside 1..n-2; one lane per channel, Thomas algorithm.  T has constant
+// diagonals, so its elimination factors c'_i (c'_1 = 1/4, c'_i =
+// 1/(4 - c'_{i-1})) are the same for every channel; in float32 they reach
+// their fixed point 2 - sqrt(3) within CP_EXACT steps, after which the
+// fixed value is used.
+constexpr int CP_EXACT = 24;
+
+__device__ __forceinline__ float elim_factor(int i)
+{
+    float q = 0.25f;
+    for (int k = 2; k <= min(i, CP_EXACT); ++k) q = 1.0f / (4.0f - q);
+    return q;
+}
+
+__global__ __launch_bounds__(256) void k_spline_adjoint(const float *__restrict__ gy,
+                                                        const float *__restrict__ ge, int gstride,
+                                                        int64_t nch, int n, float *__restrict__ drf)
+{
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nch) return;
+    const float *y = gy + c * gstride;
+    const float *e = ge + c * gstride;
+    float *o = drf + c * (int64_t)n;
+    if (n < 3) {
+        for (int k = 0; k < n; ++k) o[k] = y[k];
+        return;
+    }
+    // forward elimination, d'_i kept in o[i] (i = 1 .. n-2)
+    float cp = 0.25f, dp = e[1] * 0.25f;
+    o[1] = dp;
+    for (int i = 2; i <= n - 2; ++i) {
+        cp = (i <= CP_EXACT) ? 1.0f / (4.0f - cp) : cp;
+        dp = (e[i] - dp) * cp;
+        o[i] = dp;
+    }
+    // back substitution u_i = d'_i - c'_i u_{i+1}; entry k = i + 1 is final
+    // once u_i is known (u_{n-1} = u_n = 0)
+    const float cinf = elim_factor(CP_EXACT);
+    float u1 = 0.0f, u2 = 0.0f;  // u_{i+1}, u_{i+2}
+    for (int i = n - 2; i >= 1; --i) {
+        const float ci = (i <= CP_EXACT) ? elim_factor(i) : cinf;
+        const float ui = o[i] - ci * u1;
+        o[i + 1] = y[i + 1] + (ui - 2.0f * u1 + u2);
+        u2 = u1;
+        u1 = ui;
+    }
+    o[1] = y[1] + (u2 - 2.0f * u1);  // u_0 = 0
+    o[0] = y[0] + u1;                // u_{-1} = u_0 = 0
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(float *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+static int fill_zero(float *p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return check_launch("fill zero");
+}
+
+template <int GK>
+static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
+                        float *gapodrx, float *grf, hipStream_t s)
+{
+    const unsigned ptiles = (unsigned)((a.np + 255) / 256);
+    if (gtautx || gapodtx) {
+        BwdArgs b = a;
+        b.gtau = gtautx;
+        b.gapod = gapodtx;
+        ProfScope ps("das_bwd_tau", s);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 1>), dim3(ptiles, a.ntx), dim3(256), 0, s, b);
+        if (int rc = check_launch("das_bwd_tau")) return rc;
+    }
+    if (gtaurx || gapodrx) {
+        BwdArgs b = a;
+        b.gtau = gtaurx;
+        b.gapod = gapodrx;
+        ProfScope ps("das_bwd_tau", s);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 0>), dim3(ptiles, a.nrx), dim3(256), 0, s, b);
+        if (int rc = check_launch("das_bwd_tau")) return rc;
+    }
+    if (grf) {
+        const int64_t nch = (int64_t)a.nfr * a.ntx * a.nrx;
+        ProfScope ps("das_bwd_rf", s);
+        if (GK == GK_CUBIC) {
+            if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
+            if (int rc = fill_zero(a.ge, nch * a.gstride, s)) return rc;
+        } else {
+            if (int rc = fill_zero(grf, nch * a.nt, s)) return rc;
+        }
+        hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
+        if (int rc = check_launch("das_bwd_rf")) return rc;
+        if (GK == GK_CUBIC) {
+            hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0,
+                               s, a.gy, a.ge, a.gstride, nch, a.nt, grf);
+            if (int rc = check_launch("spline_adjoint")) return rc;
+        }
+    }
+    return 0;
+}
+
+int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
+                   float *gapodrx, float *grf, hipStream_t s)
+{
+    switch (gk) {
+    case GK_NEAREST: return launch_bwd_t<GK_NEAREST>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    case GK_LINEAR: return launch_bwd_t<GK_LINEAR>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    default: return launch_bwd_t<GK_CUBIC>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    }
+}
+
+}  // namespace pycbf

@@ -1,0 +1,132 @@
+"""Differentiable Tabbed DAS: a torch.autograd.Function whose forward is
+pycbf_das_tabbed and whose backward is pycbf_das_tabbed_bwd (hand-written
+adjoint kernels: transposed gather into LDS bins for the channel data, a
+per-(element, pixel) pass for the delays and apodisations, and the adjoint of
+the natural-spline prefilter).
+
+The forward is pycbf's Tabbed DAS (pycbf/cpu/pycbf.c:179-271 with the GPU
+layouts of pycbf/gpu/__engines__.cu:813-864); pycbf itself has no gradients.
+torch provides the device memory, the stream and the autograd graph only.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from ..base import BeamformerException
+from . import _engine as E
+
+DIFF_KINDS = ("nearest", "linear", "cubic")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _check(t, name, shape=None):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32):
+        raise BeamformerException(f"{name} must be a float32 CUDA tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise BeamformerException(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    return t.contiguous()
+
+
+def _out_shape(sumtype, nfr, ntx, nrx, np_):
+    per = {"none": (ntx, nrx, np_), "tx_only": (nrx, np_), "rx_only": (ntx, np_),
+           "tx_and_rx": (np_,)}[sumtype]
+    return (nfr,) + per
+
+
+class _TabbedDAS:
+    """Container of the non-tensor arguments (hashable-free, kept on ctx)."""
+
+    def __init__(self, t0, dt, kind, sumtype, thresh):
+        if kind not in DIFF_KINDS:
+            raise BeamformerException(f"differentiable kinds are {DIFF_KINDS}, got {kind!r}")
+        if sumtype not in E.SUMTYPE:
+            raise BeamformerException(f"unknown sumtype {sumtype!r}")
+        self.t0, self.dt, self.kind, self.sumtype = float(t0), float(dt), kind, sumtype
+        self.thresh = float(thresh)
+
+
+def _das_function():
+    torch = _torch()
+
+    class DASFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rf, tautx, apodtx, taurx, apodrx, cfg):
+            nfr, ntx, nrx, nt = rf.shape
+            np_ = tautx.shape[1]
+            info = E.rfinfo(ntx, nrx, 0, cfg.t0, cfg.dt, nt)
+            kind = E.INTERP[cfg.kind]
+            need = E.lib.pycbf_das_workspace_bytes(ctypes.byref(info), nfr, np_, kind, 1, 0)
+            ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
+            out = torch.empty(_out_shape(cfg.sumtype, nfr, ntx, nrx, np_), dtype=torch.float32,
+                              device=rf.device)
+            rc = E.lib.pycbf_das_tabbed(ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx),
+                                        E.ptr(apodtx), E.ptr(taurx), E.ptr(apodrx), None, np_,
+                                        kind, 1, None, cfg.thresh, E.SUMTYPE[cfg.sumtype],
+                                        E.ptr(out), E.ptr(ws), ws.numel(),
+                                        E.stream_of(rf.device))
+            E.check(rc, "pycbf_das_tabbed")
+            ctx.save_for_backward(rf, tautx, apodtx, taurx, apodrx)
+            ctx.cfg = cfg
+            return out
+
+        @staticmethod
+        def backward(ctx, gout):
+            rf, tautx, apodtx, taurx, apodrx = ctx.saved_tensors
+            cfg = ctx.cfg
+            need_rf, need_ttx, need_atx, need_trx, need_arx = ctx.needs_input_grad[:5]
+            gout = gout.to(torch.float32).contiguous()
+            nfr, ntx, nrx, nt = rf.shape
+            np_ = tautx.shape[1]
+            mk = lambda ok, like: torch.empty_like(like) if ok else None  # noqa: E731
+            grf, gttx, gatx = mk(need_rf, rf), mk(need_ttx, tautx), mk(need_atx, apodtx)
+            gtrx, garx = mk(need_trx, taurx), mk(need_arx, apodrx)
+            info = E.rfinfo(ntx, nrx, 0, cfg.t0, cfg.dt, nt)
+            kind = E.INTERP[cfg.kind]
+            need = E.lib.pycbf_das_bwd_workspace_bytes(ctypes.byref(info), nfr, np_, kind)
+            ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
+            rc = E.lib.pycbf_das_tabbed_bwd(
+                ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx), E.ptr(apodtx), E.ptr(taurx),
+                E.ptr(apodrx), np_, kind, cfg.thresh, E.SUMTYPE[cfg.sumtype], E.ptr(gout),
+                E.ptr(grf), E.ptr(gttx), E.ptr(gatx), E.ptr(gtrx), E.ptr(garx), E.ptr(ws),
+                ws.numel(), E.stream_of(rf.device))
+            E.check(rc, "pycbf_das_tabbed_bwd")
+            return grf, gttx, gatx, gtrx, garx, None
+
+    return DASFunction
+
+
+_FN = None
+
+
+def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype="tx_and_rx",
+               thresh=0.0):
+    """Differentiable Tabbed DAS on the GPU.
+
+    rf (ntx, nrx, nt) or (nfr, ntx, nrx, nt); tautx / apodtx (ntx, np);
+    taurx / apodrx (nrx, np); all float32 CUDA tensors, any of which may
+    require grad.  ``interp`` is 'nearest', 'linear' or 'cubic' (natural
+    spline); the output has the layout of TabbedBeamformer for ``sumtype``.
+    ``thresh`` (default 0: every sample) drops samples with
+    |apodtx * apodrx| < thresh in the forward and the adjoint alike."""
+    global _FN
+    if _FN is None:
+        _FN = _das_function()
+    cfg = _TabbedDAS(t0, dt, interp, sumtype, thresh)
+    squeeze = rf.dim() == 3
+    rf4 = rf[None] if squeeze else rf
+    if rf4.dim() != 4:
+        raise BeamformerException("rf must be (ntx, nrx, nt) or (nfr, ntx, nrx, nt)")
+    nfr, ntx, nrx, nt = rf4.shape
+    rf4 = _check(rf4, "rf")
+    np_ = tautx.shape[1] if tautx.dim() == 2 else -1
+    tautx = _check(tautx, "tautx", (ntx, np_))
+    apodtx = _check(apodtx, "apodtx", (ntx, np_))
+    taurx = _check(taurx, "taurx", (nrx, np_))
+    apodrx = _check(apodrx, "apodrx", (nrx, np_))
+    out = _FN.apply(rf4, tautx, apodtx, taurx, apodrx, cfg)
+    return out[0] if squeeze else out
