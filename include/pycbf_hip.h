@@ -192,6 +192,29 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
                          size_t workspace_bytes, void *stream);
 
+/* Straight-ray delays through a slowness map (no reference counterpart: the
+ * reference's examples/rough_sos_correction.ipynb uses an external eikonal
+ * solver).  slow (nzm, nxm) device map of 1/c with node (i, k) at
+ * (xm0 + (k+1/2) dxm, zm0 + (i+1/2) dzm), bilinear, edge-extended.
+ * Plane-wave transmits (steer radians, origin x xorig, both device (ntx)):
+ *   tautx = (q_x - xorig) sin(steer) / c0 + integral of slow along the ray
+ *   from q = (x - z tan(steer), 0) to the pixel;
+ * receive elements at (xele, 0) (device (nrx)): taurx = integral of slow from
+ * the element to the pixel.  Midpoint rule with nsteps samples per ray.
+ * pvec (np, 2) device pixel positions (x, z); tautx (ntx, np), taurx (nrx, np). */
+int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
+                     float dzm, const float *steer, const float *xorig, int ntx, float c0,
+                     const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
+                     float *tautx, float *taurx, void *stream);
+
+/* Adjoint of pycbf_ray_delays: gslow (nzm, nxm) = d loss / d slow given
+ * gtautx (ntx, np) and gtaurx (nrx, np) (either may be NULL).  gslow is
+ * written (cleared first). */
+int pycbf_ray_delays_bwd(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
+                         float dzm, const float *steer, const float *xorig, int ntx, float c0,
+                         const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
+                         const float *gtautx, const float *gtaurx, float *gslow, void *stream);
+
 /* Short-lag spatial coherence.  Replaces calc_lagn_coherence and the pixel loop
  * of examples/advanced_beamforming.ipynb:65-72,205-223 (the reference has no
  * GPU SLSC).

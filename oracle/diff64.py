@@ -101,3 +101,52 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="
     if sumtype == "rx_only":
         return s.sum(dim=2)
     return s.sum(dim=(1, 2))
+
+
+def ray_delays(slow, geo, steer, xorig, xele, pvec, c0, nsteps, chunk=4096):
+    """Straight-ray delays through a bilinear, edge-extended slowness map, in
+    float64 (restating pycbf_ray_delays): returns (tautx (ntx, np), taurx
+    (nrx, np)); differentiable with respect to ``slow`` (nzm, nxm).
+    ``geo`` has nzm, nxm, xm0, zm0, dxm, dzm."""
+    slow = slow.to(torch.float64)
+    dev = slow.device
+    steer = torch.as_tensor(steer, dtype=torch.float64, device=dev)
+    xorig = torch.as_tensor(xorig, dtype=torch.float64, device=dev)
+    xele = torch.as_tensor(xele, dtype=torch.float64, device=dev)
+    pvec = torch.as_tensor(pvec, dtype=torch.float64, device=dev)
+    nzm, nxm = geo.nzm, geo.nxm
+    t = (torch.arange(nsteps, dtype=torch.float64, device=dev) + 0.5) / nsteps
+
+    def sample(x, z):
+        u = ((x - geo.xm0) / geo.dxm - 0.5).clamp(0.0, nxm - 1.0)
+        v = ((z - geo.zm0) / geo.dzm - 0.5).clamp(0.0, nzm - 1.0)
+        k0 = torch.floor(u).clamp(max=max(nxm - 2, 0)).long()
+        i0 = torch.floor(v).clamp(max=max(nzm - 2, 0)).long()
+        wx = (u - k0) if nxm > 1 else torch.zeros_like(u)
+        wz = (v - i0) if nzm > 1 else torch.zeros_like(v)
+        k1 = (k0 + 1).clamp(max=nxm - 1)
+        i1 = (i0 + 1).clamp(max=nzm - 1)
+        f = slow.reshape(-1)
+        s00, s01 = f[i0 * nxm + k0], f[i0 * nxm + k1]
+        s10, s11 = f[i1 * nxm + k0], f[i1 * nxm + k1]
+        top = s00 + wx * (s01 - s00)
+        bot = s10 + wx * (s11 - s10)
+        return top + wz * (bot - top)
+
+    out_tx, out_rx = [], []
+    for p0 in range(0, pvec.shape[0], chunk):
+        px = pvec[p0:p0 + chunk, 0][None, :]
+        pz = pvec[p0:p0 + chunk, 1][None, :]
+        sn, cs = torch.sin(steer)[:, None], torch.cos(steer)[:, None]
+        qx = px - pz * (sn / cs)
+        L = pz / cs
+        fire = (qx - xorig[:, None]) * sn / c0
+        xs = qx[..., None] + t * (px - qx)[..., None]
+        zs = (t * pz[..., None]).expand_as(xs)
+        out_tx.append(fire + L * sample(xs, zs).mean(dim=-1))
+        qx = xele[:, None].expand(-1, px.shape[1])
+        L = torch.sqrt((px - qx) ** 2 + pz ** 2)
+        xs = qx[..., None] + t * (px - qx)[..., None]
+        zs = (t * pz[..., None]).expand_as(xs)
+        out_rx.append(L * sample(xs, zs).mean(dim=-1))
+    return torch.cat(out_tx, dim=1), torch.cat(out_rx, dim=1)

@@ -582,6 +582,73 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
     return launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
 }
 
+// ------------------------------------------------------- speed of sound ---
+static int ray_args(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
+                    float dzm, const float *steer, const float *xorig, int ntx, float c0,
+                    const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
+                    RayArgs &a)
+{
+    if (!slow || !pvec) return fail(PYCBF_EINVAL, "NULL pointer");
+    if (nzm < 1 || nxm < 1 || (int64_t)nzm * nxm > sos_max_map())
+        return fail(PYCBF_EINVAL, "slowness map must have 1..%d nodes", sos_max_map());
+    if (!(dxm > 0.0f) || !(dzm > 0.0f)) return fail(PYCBF_EINVAL, "map spacing must be > 0");
+    if (ntx < 0 || nrx < 0 || ntx + nrx < 1 || ntx + nrx > 65535)
+        return fail(PYCBF_EINVAL, "need 1..65535 elements");
+    if (ntx > 0 && (!steer || !xorig)) return fail(PYCBF_EINVAL, "steer / xorig are NULL");
+    if (nrx > 0 && !xele) return fail(PYCBF_EINVAL, "xele is NULL");
+    if (np < 1) return fail(PYCBF_EINVAL, "np must be >= 1");
+    if (nsteps < 1 || nsteps > 4096) return fail(PYCBF_EINVAL, "nsteps must be in 1..4096");
+    if (!(c0 > 0.0f)) return fail(PYCBF_EINVAL, "c0 must be > 0");
+    a.slow = slow;
+    a.g.nzm = nzm;
+    a.g.nxm = nxm;
+    a.g.xm0 = xm0;
+    a.g.zm0 = zm0;
+    a.g.rdxm = 1.0f / dxm;
+    a.g.rdzm = 1.0f / dzm;
+    a.steer = steer;
+    a.xorig = xorig;
+    a.xele = xele;
+    a.ntx = ntx;
+    a.nrx = nrx;
+    a.nsteps = nsteps;
+    a.c0 = c0;
+    a.pvec = pvec;
+    a.np = np;
+    return 0;
+}
+
+int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
+                     float dzm, const float *steer, const float *xorig, int ntx, float c0,
+                     const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
+                     float *tautx, float *taurx, void *stream)
+{
+    RayArgs a{};
+    if (int rc = ray_args(slow, nzm, nxm, xm0, zm0, dxm, dzm, steer, xorig, ntx, c0, xele, nrx,
+                          pvec, np, nsteps, a))
+        return rc;
+    if ((ntx > 0 && !tautx) || (nrx > 0 && !taurx)) return fail(PYCBF_EINVAL, "NULL output");
+    a.tautx = tautx;
+    a.taurx = taurx;
+    return launch_ray_delays(a, 0, (hipStream_t)stream);
+}
+
+int pycbf_ray_delays_bwd(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
+                         float dzm, const float *steer, const float *xorig, int ntx, float c0,
+                         const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
+                         const float *gtautx, const float *gtaurx, float *gslow, void *stream)
+{
+    RayArgs a{};
+    if (int rc = ray_args(slow, nzm, nxm, xm0, zm0, dxm, dzm, steer, xorig, ntx, c0, xele, nrx,
+                          pvec, np, nsteps, a))
+        return rc;
+    if (!gslow) return fail(PYCBF_EINVAL, "gslow is NULL");
+    a.gtautx = gtautx;
+    a.gtaurx = gtaurx;
+    a.gslow = gslow;
+    return launch_ray_delays(a, 1, (hipStream_t)stream);
+}
+
 int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                int nlag, int dmastype, int sumlags, float *out, void *stream)
 {

@@ -128,6 +128,29 @@ int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, floa
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                 int nlag, int dmastype, int sumlags, float *out, hipStream_t s);
 
+// ------------------------------------------------------- speed of sound ---
+// slowness map geometry: node (i, k) at (xm0 + (k + 1/2) / rdxm, zm0 + (i + 1/2) / rdzm)
+struct MapGeo {
+    int nzm, nxm;
+    float xm0, zm0, rdxm, rdzm;
+};
+
+struct RayArgs {
+    const float *slow;  // (nzm, nxm)
+    MapGeo g;
+    const float *steer, *xorig, *xele;
+    int ntx, nrx, nsteps;
+    float c0;
+    const float *pvec;  // (np, 2)
+    int64_t np;
+    float *tautx, *taurx;              // forward outputs
+    const float *gtautx, *gtaurx;      // adjoint inputs
+    float *gslow;                      // adjoint output (accumulated)
+};
+
+int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s);
+int sos_max_map();
+
 // ------------------------------------------------------------------ SLSC ---
 // lags is a host array; arguments validated by pycbf_slsc.
 int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,

@@ -131,6 +131,38 @@ def simulate_rf(steers, xele, nt, xs, zs, amp, fs=FS, fc=FC, bw=0.6, snr_db=30.0
     return rf.astype(np.float32)
 
 
+def simulate_rf_delays_torch(tautx, taurx, amp, nt, device, fs=FS, fc=FC, bw=0.6, snr_db=30.0,
+                             seed=0, chunk=64):
+    """torch channel data (ntx, nrx, nt) float32 from given one-way delays
+    tautx (ntx, S) and taurx (nrx, S) of S point scatterers (any medium)."""
+    import torch
+    a = _pulse_a(fc, bw)
+    tautx = torch.as_tensor(tautx, device=device).to(torch.float64)
+    taurx = torch.as_tensor(taurx, device=device).to(torch.float64)
+    amp_t = torch.as_tensor(amp, device=device).to(torch.float64)
+    ntx, nrx = tautx.shape[0], taurx.shape[0]
+    half = int(math.ceil(math.sqrt(16.0 / a) * fs)) + 1
+    k = torch.arange(-half, half + 1, device=device, dtype=torch.float64)
+    rf = torch.zeros(ntx * nrx * nt, dtype=torch.float32, device=device)
+    base = (torch.arange(ntx * nrx, device=device) * nt).view(ntx, nrx, 1, 1)
+    for s0 in range(0, amp_t.numel(), chunk):
+        tau = tautx[:, None, s0:s0 + chunk] + taurx[None, :, s0:s0 + chunk]
+        am = amp_t[s0:s0 + chunk]
+        i0 = torch.round(tau * fs)
+        idx = i0[..., None] + k
+        t = idx / fs - tau[..., None]
+        val = am[None, None, :, None] * torch.exp(-a * t * t) * torch.cos(2 * math.pi * fc * t)
+        ok = (idx >= 0) & (idx < nt)
+        flat = (base + idx.clamp(0, nt - 1).long()).reshape(-1)
+        rf.index_add_(0, flat, torch.where(ok, val, torch.zeros_like(val)).float().reshape(-1))
+    rf = rf.view(ntx, nrx, nt)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) + 7919)
+    rms = torch.sqrt(torch.mean(rf * rf)).clamp_min(1e-30)
+    rf += torch.randn(rf.shape, generator=g, device=device) * rms * 10 ** (-snr_db / 20)
+    return rf
+
+
 def simulate_rf_torch(steers, xele, nt, xs, zs, amp, device, fs=FS, fc=FC, bw=0.6,
                       snr_db=30.0, seed=0, c=C0, chunk=64):
     """torch channel data (ntx, nrx, nt) float32 on ``device`` (large workloads)."""
@@ -200,3 +232,32 @@ def workload_slsc(nrx=128, ntx=75, nx=256, nz=256, sub=8, xlim=(-19e-3, 19e-3),
     nzf = nz * sub
     _, _, pnts = grid(nx, nzf, xlim, zlim)
     return steers, xele, pw_synthetic(steers, xele, pnts), (nx, nzf)
+
+
+def sos_true_map(nzm=64, nxm=64, xlim=(-19e-3, 19e-3), zlim=(5e-3, 45e-3), seed=0,
+                 c_top=1450.0, c_bottom=1540.0, z_layer=10e-3, blobs=6, amp=20.0):
+    """Two-layer medium (c_top above z_layer, c_bottom below) with +-amp m/s
+    Gaussian blobs at seeded positions, on the (nzm, nxm) node grid covering
+    the image region; returns float32 (nzm, nxm) in m/s."""
+    rng = np.random.default_rng(seed)
+    x = xlim[0] + (np.arange(nxm) + 0.5) * (xlim[1] - xlim[0]) / nxm
+    z = zlim[0] + (np.arange(nzm) + 0.5) * (zlim[1] - zlim[0]) / nzm
+    Z, X = np.meshgrid(z, x, indexing="ij")
+    c = np.where(Z < z_layer, c_top, c_bottom).astype(np.float64)
+    for _ in range(blobs):
+        cx, cz = rng.uniform(*xlim), rng.uniform(zlim[0] + 5e-3, zlim[1])
+        r = rng.uniform(2e-3, 5e-3)
+        c += rng.choice([-1.0, 1.0]) * amp * np.exp(-((X - cx) ** 2 + (Z - cz) ** 2) / (2 * r * r))
+    return c.astype(np.float32)
+
+
+def workload_sos_geometry(ntx=31, nrx=128, nx=256, nz=256, nt=4096, xlim=(-19e-3, 19e-3),
+                          zlim=(5e-3, 45e-3), nzm=64, nxm=64):
+    """Differentiable-DAS workload geometry: 31 plane waves over +-15 deg, 128
+    elements, an nx x nz pixel grid over the image region and a 64 x 64
+    slowness map covering it."""
+    xele = linear_array(nrx)
+    steers = np.radians(np.linspace(-15, 15, ntx))
+    _, _, pnts = grid(nx, nz, xlim, zlim)
+    return dict(steers=steers, xorig=pw_origin(steers, xele), xele=xele, pnts=pnts,
+                gridshape=(nx, nz), nt=nt, xlim=xlim, zlim=zlim, nzm=nzm, nxm=nxm)

@@ -76,3 +76,72 @@ def test_autograd_matches_central_differences(kind):
             g = float(grads[ai][idx])
             scale = max(abs(fd), 1e-6 * float(grads[ai].abs().max()))
             assert abs(g - fd) <= 1e-4 * scale + 1e-9, (kind, ai, idx, g, fd)
+
+
+class _Geo:
+    def __init__(self, nzm, nxm, xm0, zm0, dxm, dzm):
+        self.nzm, self.nxm, self.xm0, self.zm0, self.dxm, self.dzm = nzm, nxm, xm0, zm0, dxm, dzm
+
+
+def _ray_case(seed=21):
+    rng = np.random.default_rng(seed)
+    geo = _Geo(8, 10, -0.01, 0.0, 0.002, 0.0025)
+    steer = np.radians([-12.0, 0.0, 7.0])
+    xele = np.linspace(-0.008, 0.008, 5)
+    xorig = np.where(steer <= 0, xele[-1], xele[0])
+    pvec = np.stack([rng.uniform(-0.008, 0.008, 30), rng.uniform(0.002, 0.019, 30)], 1)
+    return geo, steer, xorig, xele, pvec
+
+
+def test_ray_delays_homogeneous_closed_form():
+    """A uniform map 1/c gives the plane-wave delay <p - o, n>/c and the
+    element-to-pixel distance over c."""
+    geo, steer, xorig, xele, pvec = _ray_case()
+    c = 1540.0
+    slow = torch.full((geo.nzm, geo.nxm), 1.0 / c, dtype=torch.float64)
+    ttx, trx = diff64.ray_delays(slow, geo, steer, xorig, xele, pvec, c, 16)
+    exp_tx = ((pvec[None, :, 0] - xorig[:, None]) * np.sin(steer)[:, None]
+              + pvec[None, :, 1] * np.cos(steer)[:, None]) / c
+    exp_rx = np.hypot(pvec[None, :, 0] - xele[:, None], pvec[None, :, 1]) / c
+    np.testing.assert_allclose(ttx.numpy(), exp_tx, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(trx.numpy(), exp_rx, rtol=1e-12)
+
+
+def test_ray_delays_linear_slowness_is_integrated_exactly():
+    """Slowness linear in depth inside the node range: the midpoint rule of
+    the bilinear interpolant is exact, tau_rx = L * s(z_p / 2) for rays that
+    stay inside the nodes' depth range."""
+    geo = _Geo(12, 4, -0.01, -0.001, 0.005, 0.002)   # node depths 0 .. 0.022
+    zc = geo.zm0 + (np.arange(geo.nzm) + 0.5) * geo.dzm
+    s = 1 / 1540 + 2e-3 * zc                          # linear in z
+    slow = torch.as_tensor(np.repeat(s[:, None], geo.nxm, 1))
+    xele = np.array([-0.003, 0.0, 0.004])
+    pvec = np.array([[0.001, 0.012], [-0.002, 0.018], [0.0, 0.006]])
+    _, trx = diff64.ray_delays(slow, geo, np.zeros(0), np.zeros(0), xele, pvec, 1540.0, 8)
+    L = np.hypot(pvec[None, :, 0] - xele[:, None], pvec[None, :, 1])
+    # rays start at z = 0 (the first node is at z = 0): mean of s over [0, z_p]
+    exp = L * (1 / 1540 + 2e-3 * pvec[None, :, 1] / 2)
+    np.testing.assert_allclose(trx.numpy(), exp, rtol=1e-12)
+
+
+def test_ray_delays_autograd_matches_central_differences():
+    geo, steer, xorig, xele, pvec = _ray_case(22)
+    rng = np.random.default_rng(23)
+    s0 = torch.as_tensor(1 / 1540 + 2e-5 * rng.standard_normal((geo.nzm, geo.nxm)))
+    gt = torch.as_tensor(rng.standard_normal((3, 30)))
+    gr = torch.as_tensor(rng.standard_normal((5, 30)))
+
+    def L(s):
+        a, b = diff64.ray_delays(s, geo, steer, xorig, xele, pvec, 1540.0, 12)
+        return (a * gt).sum() + (b * gr).sum()
+
+    s = s0.clone().requires_grad_()
+    (g,) = torch.autograd.grad(L(s), [s])
+    for _ in range(8):
+        i, k = int(rng.integers(0, geo.nzm)), int(rng.integers(0, geo.nxm))
+        h = 1e-9
+        sp, sm = s0.clone(), s0.clone()
+        sp[i, k] += h
+        sm[i, k] -= h
+        fd = float((L(sp) - L(sm)) / (2 * h))
+        assert abs(float(g[i, k]) - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, k)
