@@ -209,6 +209,90 @@ def slsc_leg(dev, reps, with_cpu):
     return res
 
 
+def sos_leg(dev, steps):
+    """Differentiable workload: 31 plane waves over +-15 deg x 128 rx x 4096
+    samples, 256 x 256 pixels, a 64 x 64 sound-speed map (1450 m/s above
+    10 mm, 1540 m/s below, +-20 m/s blobs); channel data simulated through the
+    true map with the straight-ray model; `steps` Adam steps on the map from a
+    uniform 1540 m/s start, each a forward and a backward pass through the
+    ray-delay and DAS kernels.  Returns the JSON object of the leg."""
+    import torch
+
+    from pycbf_amd import sim
+    from pycbf_amd.gpu import _engine as E
+    from pycbf_amd.gpu.sos import SlownessGrid, SoSProblem, ray_delays
+    w = sim.workload_sos_geometry()
+    geo = SlownessGrid.covering(w["xlim"], w["zlim"], w["nxm"], w["nzm"])
+    f = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)  # noqa: E731
+    ctrue = sim.sos_true_map(w["nzm"], w["nxm"], w["xlim"], w["zlim"], seed=11)
+    xs, zs, amp = sim.scatterers(50, 10_000, w["xlim"], (w["zlim"][0], w["zlim"][1]), 11)
+    ttx, trx = ray_delays(1.0 / f(ctrue), geo, f(w["steers"]), f(w["xorig"]), f(w["xele"]),
+                          f(np.stack([xs, zs], 1)), 1540.0, 64)
+    rf = sim.simulate_rf_delays_torch(ttx, trx, amp, w["nt"], dev, seed=11)
+    prob = SoSProblem(rf=rf, steer=f(w["steers"]), xorig=f(w["xorig"]), xele=f(w["xele"]),
+                      pvec=f(w["pnts"]), gridshape=w["gridshape"], geo=geo, t0=0.0,
+                      dt=1.0 / sim.FS)
+    # the map is parametrised by a coarse 4 x 4 grid of speeds, bilinearly
+    # upsampled to the 64 x 64 map the kernels differentiate (a free 64 x 64
+    # map can raise the focus metric with non-physical maps)
+    theta = torch.full((1, 1, 4, 4), 1540.0, device=dev, requires_grad=True)
+
+    def cmap():
+        return torch.nn.functional.interpolate(theta, size=(w["nzm"], w["nxm"]), mode="bilinear",
+                                               align_corners=True)[0, 0]
+
+    with torch.no_grad():
+        e0 = float((prob.images(cmap()).sum(0) ** 2).mean())
+        ct = f(ctrue)
+        loss_true = float(prob.loss(ct, e0))
+    opt = torch.optim.Adam([theta], lr=10.0)
+    top = int(round((10e-3 - geo.zm0) / geo.dzm))      # map rows above the 10 mm layer boundary
+    rmse0 = float(torch.sqrt(((cmap().detach() - ct) ** 2).mean()))
+    top0 = float(cmap().detach()[:top].mean())
+    losses = []
+    E.lib.pycbf_profile_reset()
+    E.lib.pycbf_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = prob.loss(cmap(), e0)
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    E.lib.pycbf_profile_enable(0)
+    kern = {}
+    for k in ("ray_delays", "ray_delays_bwd", "das_gather", "das_bwd_tau", "spline_coefs"):
+        ms, n = E.profile_read(k)
+        kern[k] = round(ms / max(n, 1), 4)
+    cfin = cmap().detach()
+    rmse1 = float(torch.sqrt(((cfin - ct) ** 2).mean()))
+    top1, toptrue = float(cfin[:top].mean()), float(ct[:top].mean())
+    ntx, nrx = len(w["steers"]), len(w["xele"])
+    samples = ntx * nrx * w["gridshape"][0] * w["gridshape"][1]
+    log(f"sos leg: {steps} Adam steps {wall * 1e3 / steps:.2f} ms/step, loss {losses[0]:.5f} -> "
+        f"{losses[-1]:.5f} (true map {loss_true:.5f}), map rmse {rmse0:.2f} -> {rmse1:.2f} m/s, "
+        f"top layer {top0:.1f} -> {top1:.1f} (true {toptrue:.1f}) m/s")
+    del rf, prob
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"differentiable DAS for speed of sound: {ntx} PW x {nrx} rx x {w['nt']} "
+                    f"samples, {w['gridshape'][0]}x{w['gridshape'][1]} px, "
+                    f"{w['nzm']}x{w['nxm']} map (4x4 speed parameters), straight-ray delays, cubic, "
+                    f"{steps} Adam steps",
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "fwd_bwd_gsamples_per_s": round(samples / (wall / steps) / 1e9, 2),
+        "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
+        "loss_at_true_map": round(loss_true, 6),
+        "map_rmse_m_per_s": [round(rmse0, 3), round(rmse1, 3)],
+        "top_layer_mean_m_per_s": {"start": round(top0, 2), "end": round(top1, 2),
+                                   "true": round(toptrue, 2)},
+        "kernel_ms": kern,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +304,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=12,
                     help="frames timed for the CPU baseline (about 10-30 s of host work)")
     ap.add_argument("--no-slsc", action="store_true", help="skip the coherence leg")
+    ap.add_argument("--no-sos", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--sos-steps", type=int, default=50)
     ap.add_argument("--slsc-reps", type=int, default=20)
     ap.add_argument("--data", choices=["sim", "noise"], default="sim",
                     help="noise: seeded Gaussian RF of the same shape (profiling runs)")
@@ -303,6 +389,7 @@ def main():
         del rf
         torch.cuda.empty_cache()
         slsc_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
+    sos_res = None if args.no_sos else sos_leg(dev, args.sos_steps)
 
     line = {
         "metric": METRIC,
@@ -330,6 +417,7 @@ def main():
                       "spline_coefs": round(s_ms / max(s_n, 1), 4),
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
         "slsc": slsc_res,
+        "sos": sos_res,
     }
     print(json.dumps(line), flush=True)
 

@@ -207,13 +207,17 @@ int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, 
                      const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
                      float *tautx, float *taurx, void *stream);
 
+/* Workspace bytes of pycbf_ray_delays_bwd (partial maps of its workgroups). */
+size_t pycbf_ray_delays_bwd_workspace_bytes(int nzm, int nxm);
+
 /* Adjoint of pycbf_ray_delays: gslow (nzm, nxm) = d loss / d slow given
  * gtautx (ntx, np) and gtaurx (nrx, np) (either may be NULL).  gslow is
- * written (cleared first). */
+ * written. */
 int pycbf_ray_delays_bwd(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                          float dzm, const float *steer, const float *xorig, int ntx, float c0,
                          const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
-                         const float *gtautx, const float *gtaurx, float *gslow, void *stream);
+                         const float *gtautx, const float *gtaurx, float *gslow, void *workspace,
+                         size_t workspace_bytes, void *stream);
 
 /* Short-lag spatial coherence.  Replaces calc_lagn_coherence and the pixel loop
  * of examples/advanced_beamforming.ipynb:65-72,205-223 (the reference has no

@@ -633,19 +633,31 @@ int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, 
     return launch_ray_delays(a, 0, (hipStream_t)stream);
 }
 
+size_t pycbf_ray_delays_bwd_workspace_bytes(int nzm, int nxm)
+{
+    if (nzm < 1 || nxm < 1) return 0;
+    return ray_bwd_workspace_bytes(nzm, nxm);
+}
+
 int pycbf_ray_delays_bwd(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                          float dzm, const float *steer, const float *xorig, int ntx, float c0,
                          const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
-                         const float *gtautx, const float *gtaurx, float *gslow, void *stream)
+                         const float *gtautx, const float *gtaurx, float *gslow, void *workspace,
+                         size_t workspace_bytes, void *stream)
 {
     RayArgs a{};
     if (int rc = ray_args(slow, nzm, nxm, xm0, zm0, dxm, dzm, steer, xorig, ntx, c0, xele, nrx,
                           pvec, np, nsteps, a))
         return rc;
     if (!gslow) return fail(PYCBF_EINVAL, "gslow is NULL");
+    const size_t need = ray_bwd_workspace_bytes(nzm, nxm);
+    if (!workspace || workspace_bytes < need)
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes);
     a.gtautx = gtautx;
     a.gtaurx = gtaurx;
     a.gslow = gslow;
+    a.gpart = (float *)workspace;
     return launch_ray_delays(a, 1, (hipStream_t)stream);
 }
 

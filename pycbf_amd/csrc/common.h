@@ -145,11 +145,13 @@ struct RayArgs {
     int64_t np;
     float *tautx, *taurx;              // forward outputs
     const float *gtautx, *gtaurx;      // adjoint inputs
-    float *gslow;                      // adjoint output (accumulated)
+    float *gslow;                      // adjoint output
+    float *gpart;                      // adjoint scratch: partial maps of the workgroups
 };
 
 int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s);
 int sos_max_map();
+size_t ray_bwd_workspace_bytes(int nzm, int nxm);
 
 // ------------------------------------------------------------------ SLSC ---
 // lags is a host array; arguments validated by pycbf_slsc.

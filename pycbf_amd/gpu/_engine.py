@@ -87,9 +87,10 @@ _SIGS = {
                               ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_ray_delays": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp, ct.c_int,
                           _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp], ct.c_int),
+    "pycbf_ray_delays_bwd_workspace_bytes": ([ct.c_int, ct.c_int], _sz),
     "pycbf_ray_delays_bwd": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp,
                               ct.c_int, _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp,
-                              _vp], ct.c_int),
+                              _vp, _sz, _vp], ct.c_int),
     "pycbf_profile_enable": ([ct.c_int], None),
     "pycbf_profile_reset": ([], None),
     "pycbf_profile_read": ([ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)],

@@ -74,11 +74,14 @@ def _ray_function():
             gslow = torch.empty_like(slow)
             gt = None if gtautx is None else gtautx.to(torch.float32).contiguous()
             gr = None if gtaurx is None else gtaurx.to(torch.float32).contiguous()
+            need = E.lib.pycbf_ray_delays_bwd_workspace_bytes(geo.nzm, geo.nxm)
+            ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=slow.device)
             rc = E.lib.pycbf_ray_delays_bwd(E.ptr(slow), geo.nzm, geo.nxm, geo.xm0, geo.zm0,
                                             geo.dxm, geo.dzm, E.ptr(steer), E.ptr(xorig),
                                             steer.numel(), ctx.c0, E.ptr(xele), xele.numel(),
                                             E.ptr(pvec), pvec.shape[0], ctx.nsteps, E.ptr(gt),
-                                            E.ptr(gr), E.ptr(gslow), E.stream_of(slow.device))
+                                            E.ptr(gr), E.ptr(gslow), E.ptr(ws), ws.numel(),
+                                            E.stream_of(slow.device))
             E.check(rc, "pycbf_ray_delays_bwd")
             return gslow, None, None, None, None, None, None, None
 
@@ -153,6 +156,8 @@ class SoSProblem:
     nsteps: int = 64
     kwin: int = 5
     interp: str = "cubic"
+    energy_weight: float = 0.1
+    smooth_weight: float = 0.0
 
     def images(self, cmap):
         """Per-receive-channel images (nrx, nx, nz) for the sound-speed map
@@ -178,5 +183,11 @@ class SoSProblem:
 
     def loss(self, cmap, energy0):
         """Focus loss of the map (the kernels run in float32; the scalar loss
-        on their images is evaluated in float64)."""
-        return focus_loss(self.images(cmap).double(), energy0, self.kwin)
+        on their images is evaluated in float64) plus smooth_weight times the
+        mean squared neighbour difference of the map in m/s."""
+        loss = focus_loss(self.images(cmap).double(), energy0, self.kwin, self.energy_weight)
+        if self.smooth_weight > 0.0:
+            c = cmap.double()
+            rough = ((c[1:] - c[:-1]) ** 2).mean() + ((c[:, 1:] - c[:, :-1]) ** 2).mean()
+            loss = loss + self.smooth_weight * rough
+        return loss
