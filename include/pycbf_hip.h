@@ -251,6 +251,57 @@ int pycbf_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t nz
 int pycbf_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t nz, int kwin,
                          int kofs, int dsfx, int dsfz, uint64_t *sel, void *stream);
 
+/* ------------------------------------------------------------ batches ---
+ * Ragged batches of Synthetic (2-D, plane-wave packs from pycbf_synthetic_pack)
+ * cubic DAS and SLSC requests in one persistent gather launch (no reference
+ * counterpart: pycbf beamforms one configuration per call,
+ * pycbf/gpu/__core_bf_classes__.py:68-118).  All requests share nt and dt.
+ * The channel data of request r is the r-th block of one device buffer
+ * rf_all (requests in order, each (ntx, nrx, nt)); DAS outputs must be laid
+ * out back to back in request order. */
+#define PYCBF_BATCH_DAS 0  /* image summed over transmits and receives, out (nx*nz) */
+#define PYCBF_BATCH_SLSC 1 /* per-receive images into imrx (nrx, nx*nz), then SLSC into out */
+
+typedef struct pycbf_batch_req {
+    int32_t kind, ntx, nrx, nx, nz, nt; /* x-major grid of nx * nz pixels */
+    float t0, dt, c0, thresh;
+    const float *txpack, *rxpack; /* device packs */
+    const float *pvec;            /* device (nx*nz, 2) pixel positions */
+    float *out;                   /* device output (DAS image, or SLSC image) */
+    float *imrx;                  /* device (nrx, nx*nz) scratch of SLSC requests */
+    const uint64_t *sel;          /* SLSC: pycbf_slsc_selection bits or NULL */
+    int32_t kwin, kofs, dsfx, dsfz, nlag, precision; /* SLSC arguments as pycbf_slsc */
+    int32_t lags[64];
+} pycbf_batch_req;
+
+typedef struct pycbf_batch pycbf_batch; /* host-side plan of a batch */
+
+/* Validates the requests and builds the host plan (copied from reqs). */
+int pycbf_batch_create(const pycbf_batch_req *reqs, int nreq, pycbf_batch **batch);
+void pycbf_batch_destroy(pycbf_batch *batch);
+/* Device workspace bytes a batch needs (spline tables of every channel, the
+ * request table, the work items of the largest configuration, counters). */
+size_t pycbf_batch_workspace_bytes(const pycbf_batch *batch);
+/* Picks pixel tile (8, 16 or 32 square) and receive chunk (16, 32 or 64) and
+ * uploads the request table and the cost-ordered work items; synchronises. */
+int pycbf_batch_configure(pycbf_batch *batch, int tile, int rchunk, void *workspace,
+                          size_t workspace_bytes, void *stream);
+/* Enqueues one run (kernels only, capturable in a graph): counter / output
+ * reset, spline prefilter of all channels, the persistent gather, and the
+ * SLSC of the SLSC requests. */
+int pycbf_batch_run(pycbf_batch *batch, const float *rf_all, void *workspace,
+                    size_t workspace_bytes, void *stream);
+/* Times the persistent gather for every (tile, rchunk) pair (reps runs each,
+ * after one full run that builds the tables), configures the fastest and
+ * returns it; times_ms (9 entries, tile-major) may be NULL.  Synchronises. */
+int pycbf_batch_autotune(pycbf_batch *batch, const float *rf_all, int reps, void *workspace,
+                         size_t workspace_bytes, void *stream, int *best_tile, int *best_rchunk,
+                         float *times_ms);
+/* Completion time of every request, in ms after the start of the last run
+ * (device wall clock); synchronous readback. */
+int pycbf_batch_latencies(const pycbf_batch *batch, const void *workspace,
+                          size_t workspace_bytes, double *latency_ms);
+
 /* Per-kernel timing: when enabled, instrumented launches record HIP events on
  * their stream; pycbf_profile_read synchronises them and returns the summed
  * duration (ms) and launch count of kernel `kernel` ("das_gather",

@@ -52,6 +52,18 @@ class RFInfo(ct.Structure):
 
 assert ct.sizeof(RFInfo) == 32
 
+
+class BatchReq(ct.Structure):
+    """struct pycbf_batch_req (include/pycbf_hip.h)."""
+    _fields_ = [("kind", ct.c_int32), ("ntx", ct.c_int32), ("nrx", ct.c_int32),
+                ("nx", ct.c_int32), ("nz", ct.c_int32), ("nt", ct.c_int32),
+                ("t0", ct.c_float), ("dt", ct.c_float), ("c0", ct.c_float),
+                ("thresh", ct.c_float), ("txpack", ct.c_void_p), ("rxpack", ct.c_void_p),
+                ("pvec", ct.c_void_p), ("out", ct.c_void_p), ("imrx", ct.c_void_p),
+                ("sel", ct.c_void_p), ("kwin", ct.c_int32), ("kofs", ct.c_int32),
+                ("dsfx", ct.c_int32), ("dsfz", ct.c_int32), ("nlag", ct.c_int32),
+                ("precision", ct.c_int32), ("lags", ct.c_int32 * 64)]
+
 _vp = ct.c_void_p
 _i32 = ct.c_int32
 _i64 = ct.c_int64
@@ -91,6 +103,13 @@ _SIGS = {
     "pycbf_ray_delays_bwd": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp,
                               ct.c_int, _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp,
                               _vp, _sz, _vp], ct.c_int),
+    "pycbf_batch_create": ([_vp, ct.c_int, _vp], ct.c_int),
+    "pycbf_batch_destroy": ([_vp], None),
+    "pycbf_batch_workspace_bytes": ([_vp], _sz),
+    "pycbf_batch_configure": ([_vp, ct.c_int, ct.c_int, _vp, _sz, _vp], ct.c_int),
+    "pycbf_batch_run": ([_vp, _vp, _vp, _sz, _vp], ct.c_int),
+    "pycbf_batch_autotune": ([_vp, _vp, ct.c_int, _vp, _sz, _vp, _vp, _vp, _vp], ct.c_int),
+    "pycbf_batch_latencies": ([_vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_profile_enable": ([ct.c_int], None),
     "pycbf_profile_reset": ([], None),
     "pycbf_profile_read": ([ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)],
