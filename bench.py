@@ -293,6 +293,72 @@ def sos_leg(dev, steps):
     }
 
 
+def batch_leg(dev, runs, speckle):
+    """Ragged batch: 64 seeded requests, grid side in {128, 192, 256, 384, 512},
+    Ntx in {1, 11, 31, 75}, Nrx in {64, 128, 192, 256}, Nt = 4096; cubic DAS
+    for 48 requests and SLSC (lags 1..20) for 16; one persistent gather launch
+    per run, autotuned tile / receive chunk, the run captured in a HIP graph
+    and replayed.  Returns the JSON object of the leg."""
+    import torch
+
+    from pycbf_amd import sim
+    from pycbf_amd.gpu import _engine as E
+    from pycbf_amd.gpu.batch import BatchRequest, RaggedBatch
+    rng = np.random.default_rng(2024)
+    reqs, specs = [], []
+    kinds = ["das"] * 48 + ["slsc"] * 16
+    rng.shuffle(kinds)
+    for k in range(64):
+        side = int(rng.choice([128, 192, 256, 384, 512]))
+        ntx = int(rng.choice([1, 11, 31, 75]))
+        nrx = int(rng.choice([64, 128, 192, 256]))
+        xele = sim.linear_array(nrx)
+        steers = np.radians(np.linspace(-16, 16, ntx)) if ntx > 1 else np.zeros(1)
+        _, _, pnts = sim.grid(side, side, (-19e-3, 19e-3), (2e-3, 50e-3))
+        reqs.append(BatchRequest(params=sim.pw_synthetic(steers, xele, pnts),
+                                 gridshape=(side, side), kind=kinds[k], lags=tuple(range(1, 21)),
+                                 kwin=5, precision="fp16"))
+        specs.append((steers, xele, ntx, nrx, side))
+    batch = RaggedBatch(reqs, nt=NT, dt=1.0 / 40e6, device=dev)
+    for k, (steers, xele, ntx, nrx, side) in enumerate(specs):
+        xs, zs, amp = sim.scatterers(50, speckle, (-19e-3, 19e-3), (2e-3, 50e-3), 5000 + k)
+        batch.rf_views[k].copy_(sim.simulate_rf_torch(steers, xele, NT, xs, zs, amp, dev,
+                                                      seed=5000 + k))
+    torch.cuda.synchronize()
+    log(f"batch leg: 64 requests, {batch.samples / 1e9:.2f} G samples per run")
+    tile, rchunk, grid = batch.autotune(reps=2)
+    log(f"autotune: tile {tile} rchunk {rchunk} ({grid[(tile, rchunk)]:.2f} ms gather)")
+    batch.capture()
+    batch.run()
+    torch.cuda.synchronize()
+    E.lib.pycbf_profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(runs):
+        batch.run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / runs
+    lat = batch.latencies_ms()
+    log(f"batch: {ms:.2f} ms/run, {batch.samples / (ms * 1e-3) / 1e9:.1f} Gsamples/s, latency p50 "
+        f"{np.percentile(lat, 50):.2f} p99 {np.percentile(lat, 99):.2f} ms")
+    res = {
+        "workload": "ragged batch of 64 seeded requests (48 cubic DAS, 16 SLSC lags 1..20), grids "
+                    "128..512 square, Ntx 1..75, Nrx 64..256, Nt 4096; one persistent gather "
+                    "launch per run, HIP-graph replay",
+        "ms_per_run": round(ms, 3),
+        "gsamples_per_s": round(batch.samples / (ms * 1e-3) / 1e9, 2),
+        "samples_per_run": int(batch.samples),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+        "autotune": {"tile": tile, "rchunk": rchunk,
+                     "gather_ms": {f"{t}x{t}/{c}": round(v, 3) for (t, c), v in grid.items()}},
+        "data": f"seeded channel data, 50 points + {speckle} speckle scatterers per request",
+    }
+    del batch
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +372,9 @@ def main():
     ap.add_argument("--no-slsc", action="store_true", help="skip the coherence leg")
     ap.add_argument("--no-sos", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--sos-steps", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch leg")
+    ap.add_argument("--batch-runs", type=int, default=5)
+    ap.add_argument("--batch-speckle", type=int, default=2000)
     ap.add_argument("--slsc-reps", type=int, default=20)
     ap.add_argument("--data", choices=["sim", "noise"], default="sim",
                     help="noise: seeded Gaussian RF of the same shape (profiling runs)")
@@ -390,6 +459,7 @@ def main():
         torch.cuda.empty_cache()
         slsc_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
     sos_res = None if args.no_sos else sos_leg(dev, args.sos_steps)
+    batch_res = None if args.no_batch else batch_leg(dev, args.batch_runs, args.batch_speckle)
 
     line = {
         "metric": METRIC,
@@ -418,6 +488,7 @@ def main():
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
         "slsc": slsc_res,
         "sos": sos_res,
+        "batch": batch_res,
     }
     print(json.dumps(line), flush=True)
 
