@@ -30,11 +30,13 @@ struct __attribute__((aligned(4))) F4u {
     float v[4];
 };
 
-// LDS staging area of the spline sweeps: 64 rows (channels) x 72 floats.
-// Forward: columns [0,32) hold the y tile, [36,68) the u tile.  Backward:
-// y tile, u tile, then [0,64) the interleaved (y, h*y2) output entries.
-constexpr int SPL_PITCH = 72;
-constexpr int SPL_UOFF = 36;
+
+// LDS staging tile of the spline sweeps: 64 rows (channels) x 32 floats at a
+// pitch of 36 floats, so that rows stay 16-byte aligned and both the
+// row-per-lane and the 8-lanes-per-row float4 accesses are free of bank
+// conflicts.  The one tile holds, in turn, the y tile and the u tile
+// (forward), and the u tile and the output tile (back substitution).
+constexpr int SPL_PITCH = 36;
 typedef float SplRow[SPL_PITCH];
 
 // A 64-row x 32-column tile moves between global memory and registers in
@@ -44,17 +46,31 @@ struct TileRegs {
     F4u q[8];
 };
 
-__device__ __forceinline__ void tile_fetch(const float *base, int64_t ld, int nrow, int s, int ncol,
+// Addresses are a uniform base (the workgroup's first row) plus a 32-bit byte
+// offset per lane: 64 table rows of at most SPL_MAX_NT + 1 entries (8 bytes
+// each) stay below 2^32 bytes.
+constexpr int SPL_MAX_NT = (1 << 22) - 1;
+__device__ __forceinline__ const float *at(const float *base, uint32_t elems)
+{
+    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + 4u * elems);
+}
+__device__ __forceinline__ float *at(float *base, uint32_t elems)
+{
+    return reinterpret_cast<float *>(reinterpret_cast<char *>(base) + 4u * elems);
+}
+
+__device__ __forceinline__ void tile_fetch(const float *base, int ld, int nrow, int s, int ncol,
                                            TileRegs &t)
 {
     const int lane = threadIdx.x;
     const int c = 4 * (lane & 7);
+    const uint32_t o = (uint32_t)((lane >> 3) * ld + s + c);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int r = 8 * q + (lane >> 3);
         F4u w{{0.0f, 0.0f, 0.0f, 0.0f}};
         if (r < nrow) {
-            const float *src = base + (int64_t)r * ld + s + c;
+            const float *src = at(base, o + (uint32_t)(8 * q * ld));
             if (s + c + 4 <= ncol) {
                 w = *reinterpret_cast<const F4u *>(src);
             } else {
@@ -67,27 +83,28 @@ __device__ __forceinline__ void tile_fetch(const float *base, int64_t ld, int nr
     }
 }
 
-__device__ __forceinline__ void tile_to_lds(const TileRegs &t, SplRow *S, int coff)
+__device__ __forceinline__ void tile_to_lds(const TileRegs &t, SplRow *S)
 {
     const int lane = threadIdx.x;
-    const int c = coff + 4 * (lane & 7);
+    const int c = 4 * (lane & 7);
 #pragma unroll
     for (int q = 0; q < 8; ++q)
         *reinterpret_cast<float4 *>(&S[8 * q + (lane >> 3)][c]) =
             make_float4(t.q[q].v[0], t.q[q].v[1], t.q[q].v[2], t.q[q].v[3]);
 }
 
-__device__ __forceinline__ void lds_to_global(const SplRow *S, int coff, float *base, int64_t ld,
-                                              int nrow, int s, int ncol)
+__device__ __forceinline__ void lds_to_global(const SplRow *S, float *base, int ld, int nrow, int s,
+                                              int ncol)
 {
     const int lane = threadIdx.x;
     const int c = 4 * (lane & 7);
+    const uint32_t o = (uint32_t)((lane >> 3) * ld + s + c);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int r = 8 * q + (lane >> 3);
         if (r >= nrow) continue;
-        const float4 v = *reinterpret_cast<const float4 *>(&S[r][coff + c]);
-        float *dst = base + (int64_t)r * ld + s + c;
+        const float4 v = *reinterpret_cast<const float4 *>(&S[r][c]);
+        float *dst = at(base, o + (uint32_t)(8 * q * ld));
         if (s + c + 4 <= ncol) {
             F4u w{{v.x, v.y, v.z, v.w}};
             *reinterpret_cast<F4u *>(dst) = w;
@@ -100,6 +117,27 @@ __device__ __forceinline__ void lds_to_global(const SplRow *S, int coff, float *
     }
 }
 
+// lane's row of the staging tile <-> 32 registers
+__device__ __forceinline__ void row_read(const SplRow *S, float (&v)[32])
+{
+#pragma unroll
+    for (int c = 0; c < 32; c += 4) {
+        const float4 w = *reinterpret_cast<const float4 *>(&S[threadIdx.x][c]);
+        v[c] = w.x;
+        v[c + 1] = w.y;
+        v[c + 2] = w.z;
+        v[c + 3] = w.w;
+    }
+}
+
+__device__ __forceinline__ void row_write(SplRow *S, const float (&v)[32])
+{
+#pragma unroll
+    for (int c = 0; c < 32; c += 4)
+        *reinterpret_cast<float4 *>(&S[threadIdx.x][c]) =
+            make_float4(v[c], v[c + 1], v[c + 2], v[c + 3]);
+}
+
 // One wave per 64 channels, one lane per channel for the recurrences.  Tiles
 // of 32 samples move through LDS (coalesced global traffic, transposed to one
 // row per lane) with the next tile prefetched into registers.  Per tile the
@@ -108,159 +146,190 @@ __device__ __forceinline__ void lds_to_global(const SplRow *S, int coff, float *
 // per lane), so only u_i = (g_i - 0.5*u_{i-1}) / p_i remains on the serial
 // chain.  Operations and their order are those of tie_knots1D_fixed
 // (pycbf/cpu/pycbf.c:126-141): y2 is bit-identical.
+//
+// Tile cb of the forward sweep (samples s = 32*cb .. s+31 in LDS) computes
+// u_i for i = s-1 .. s+30, whose right neighbour y[i+1] lies in the same
+// tile; y[s-1] and the slope ending there are carried in registers.  The
+// scratch row therefore holds U[i+1] = u_i (U[0] = 0), so that each tile's
+// u values land in one aligned 32-sample tile of the scratch row, and the
+// back substitution of tile cb produces y2[s-1 .. s+30].
+//
+// Resources: 64-thread workgroups, 9.3 KB of LDS and at most 168 VGPRs
+// (3 waves per SIMD): the 2400 waves of a 16-frame, 75 x 128-channel batch
+// are all resident at once on 1024 SIMDs instead of running in two rounds.
 template <int MODE>  // 0: (y, dx^2/6*y2) gather table, 1: y2
-__global__ __launch_bounds__(64) void k_spline(const float *__restrict__ rf, int64_t nch, int nt,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_spline(const float *__restrict__ rf, int64_t nch, int nt,
                                                float dx, float *__restrict__ uscr,
                                                void *__restrict__ outp)
 {
     constexpr int CH = SPL_CHUNK;
     static_assert(CH == 32, "tile helpers assume 32-sample tiles");
     __shared__ __align__(16) SplRow S[64];
+    __shared__ float y2f[32];  // forward coefficients; y2f[i] = y2f[31] for i >= 7
 
     const int lane = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * 64;
     const int nrow = (int)((nch - c0) < 64 ? (nch - c0) : 64);
-    const bool live = lane < nrow;
     const float *ybase = rf + c0 * (int64_t)nt;
     float *ubase = uscr + c0 * (int64_t)nt;
 
-    float y2f[32];
+    float y2f_inf;
     {
-        float v = 0.0f;
-        y2f[0] = 0.0f;
+        // y2f[i] = (0.5 - 1) / (0.5*y2f[i-1] + 2), y2f[0] = 0 (pycbf.c:127-133)
+        float v = 0.0f, mine = 0.0f;
 #pragma unroll
         for (int i = 1; i < 32; ++i) {
             const float p = 0.5f * v + 2.0f;
             v = (0.5f - 1.0f) / p;
-            y2f[i] = v;
+            if (lane == i) mine = v;
         }
+        if (lane < 32) y2f[lane] = mine;
+        y2f_inf = v;
     }
-    const float p_inf = 0.5f * y2f[31] + 2.0f;  // p_i for every i >= 32
+    const float p_inf = 0.5f * y2f_inf + 2.0f;  // p_i for every i >= 8
     const int nchunk = (nt + CH - 1) / CH;
 
     // ---- forward sweep ---------------------------------------------------
     TileRegs pf;
     tile_fetch(ybase, nt, nrow, 0, nt, pf);
-    tile_to_lds(pf, S, 0);
+    tile_to_lds(pf, S);
     if (nchunk > 1) tile_fetch(ybase, nt, nrow, CH, nt, pf);
     __syncthreads();
-    // slope (y[i] - y[i-1])/dx of the previous step: the reference evaluates
-    // both slopes at every step, and this one equals the previous step's
-    // forward slope (same operands, same correctly rounded division)
-    float d_prev = 0.0f, u_prev = 0.0f;
+    // carried: y[s-1], the slope (y[s-1] - y[s-2])/dx, and u_{s-2}.  The
+    // reference evaluates both slopes at every step; the backward one equals
+    // the previous step's forward slope (same operands, same rounding).
+    float y_prev = 0.0f, d_prev = 0.0f, u_prev = 0.0f;
     for (int cb = 0; cb < nchunk; ++cb) {
         const int s = cb * CH;
-        float y[CH + 1];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) y[c] = S[lane][c];
-        y[CH] = (live && s + CH < nt) ? ybase[(int64_t)lane * nt + s + CH] : 0.0f;
-        float g[CH];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            const float d = (y[c + 1] - y[c]) / dx;
-            const float jump = d - d_prev;
-            g[c] = 6.0f * jump / (2.0f * dx);
-            d_prev = d;
-        }
-        float u[CH];
-        if (s >= 32 && s + CH <= nt - 1) {
-            // interior tile: p is the fixed point and every i is in [1, nt-2]
+        float v[CH];
+        row_read(S, v);
+        // v[c] <- g_{s-1+c}
+        {
+            float yl = y_prev;
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                u_prev = (g[c] - 0.5f * u_prev) / p_inf;
-                u[c] = u_prev;
+                const float yr = v[c];
+                const float d = (yr - yl) / dx;
+                const float jump = d - d_prev;
+                v[c] = 6.0f * jump / (2.0f * dx);
+                d_prev = d;
+                yl = yr;
+            }
+            y_prev = yl;
+        }
+        // v[c] <- u_{s-1+c} (zero outside 1 <= i <= nt-2)
+        if (s >= 32 && s + CH <= nt) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                u_prev = (v[c] - 0.5f * u_prev) / p_inf;
+                v[c] = u_prev;
             }
         } else {
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                const int i = s + c;
+                const int i = s - 1 + c;
                 float ui = 0.0f;
                 if (i >= 1 && i <= nt - 2) {
                     const float p = 0.5f * y2f[(i - 1) < 31 ? (i - 1) : 31] + 2.0f;
-                    ui = (g[c] - 0.5f * u_prev) / p;
+                    ui = (v[c] - 0.5f * u_prev) / p;
                     u_prev = ui;
                 }
-                u[c] = ui;
+                v[c] = ui;
             }
         }
-#pragma unroll
-        for (int c = 0; c < CH; c += 4)
-            *reinterpret_cast<float4 *>(&S[lane][SPL_UOFF + c]) =
-                make_float4(u[c], u[c + 1], u[c + 2], u[c + 3]);
-        __syncthreads();  // u tile complete; every lane is done with the y tile
-        lds_to_global(S, SPL_UOFF, ubase, nt, nrow, s, nt);
+        row_write(S, v);  // the lane's own row: no other lane read it
+        __syncthreads();  // u tile complete
+        lds_to_global(S, ubase, nt, nrow, s, nt);
+        __syncthreads();  // u tile read out
         if (cb + 1 < nchunk) {
-            tile_to_lds(pf, S, 0);
+            tile_to_lds(pf, S);
             if (cb + 2 < nchunk) tile_fetch(ybase, nt, nrow, s + 2 * CH, nt, pf);
         }
-        __syncthreads();  // next y tile complete; u tile read out
+        __syncthreads();  // next y tile complete
     }
-    // u rows are read back below by the lanes that wrote them
+    // the scratch rows are read back below by the lanes that wrote them
     __threadfence();
 
     // ---- back substitution: y2[k] = y2f[k]*y2[k+1] + u[k] ------------------
     const float h = (1.0f / 6.0f) * (dx * dx);
-    float y2_next = 0.0f;  // y2[k+1]; zero at the natural end and beyond
+    // y2[s+31] at the top of tile cb: zero for the last tile (s+31 >= nt-1)
+    float y2_carry = 0.0f;
     const int slast = (nchunk - 1) * CH;
     TileRegs py, pu;
     if (MODE == 0) tile_fetch(ybase, nt, nrow, slast, nt, py);
     tile_fetch(ubase, nt, nrow, slast, nt, pu);
-    if (MODE == 0 && live) {
+    if (MODE == 0 && lane < nrow) {
         float2 *tab = static_cast<float2 *>(outp) + (c0 + lane) * (int64_t)(nt + 1);
         tab[nt] = make_float2(0.0f, 0.0f);
     }
     for (int cb = nchunk - 1; cb >= 0; --cb) {
         const int s = cb * CH;
-        if (MODE == 0) tile_to_lds(py, S, 0);
-        tile_to_lds(pu, S, SPL_UOFF);
+        tile_to_lds(pu, S);  // U[s+c] = u_{s-1+c}
         __syncthreads();
-        if (cb > 0) {
-            if (MODE == 0) tile_fetch(ybase, nt, nrow, s - CH, nt, py);
-            tile_fetch(ubase, nt, nrow, s - CH, nt, pu);
-        }
-        float yr[CH], out[CH];
+        if (cb > 0) tile_fetch(ubase, nt, nrow, s - CH, nt, pu);
+        float v[CH];
+        row_read(S, v);
+        // k = s-1+c from c = 31 down to 0; v[c] <- output entry s+c, i.e.
+        // y2[s+c] (y2[s+31] is the carry, y2[s-1] the next tile's carry)
+        float nxt = y2_carry;
+        if (s >= 32 && s + CH <= nt) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c) yr[c] = (MODE == 0) ? S[lane][c] : 0.0f;
-#pragma unroll
-        for (int c = CH - 1; c >= 0; --c) {
-            const int k = s + c;
-            float y2k = 0.0f;
-            if (k < nt - 1) y2k = y2f[k < 31 ? k : 31] * y2_next + S[lane][SPL_UOFF + c];
-            if (k < nt) y2_next = y2k;
-            out[c] = (MODE == 0) ? h * y2k : y2k;
-        }
-        __syncthreads();  // every lane has read its y and u rows
-        if (MODE == 0) {
-#pragma unroll
-            for (int c = 0; c < CH; c += 2)
-                *reinterpret_cast<float4 *>(&S[lane][2 * c]) =
-                    make_float4(yr[c], out[c], yr[c + 1], out[c + 1]);
-            __syncthreads();
-            // row r: 32 entries = 256 contiguous bytes, 16 lanes per row
-            float2 *tab = static_cast<float2 *>(outp) + c0 * (int64_t)(nt + 1);
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int r = 4 * q + (lane >> 4), e = 2 * (lane & 15);
-                if (r >= nrow) continue;
-                float2 *dst = tab + (int64_t)r * (nt + 1) + s + e;
-                const float4 v = *reinterpret_cast<const float4 *>(&S[r][2 * e]);
-                if (s + e + 2 <= nt) {
-                    F4u w{{v.x, v.y, v.z, v.w}};
-                    *reinterpret_cast<F4u *>(dst) = w;
-                } else if (s + e < nt) {
-                    dst[0] = make_float2(v.x, v.y);
-                }
+            for (int c = CH - 1; c >= 0; --c) {
+                const float y2k = y2f_inf * nxt + v[c];
+                v[c] = nxt;
+                nxt = y2k;
             }
         } else {
 #pragma unroll
-            for (int c = 0; c < CH; c += 4)
-                *reinterpret_cast<float4 *>(&S[lane][c]) =
-                    make_float4(out[c], out[c + 1], out[c + 2], out[c + 3]);
-            __syncthreads();
-            float *y2 = static_cast<float *>(outp) + c0 * (int64_t)nt;
-            lds_to_global(S, 0, y2, nt, nrow, s, nt);
+            for (int c = CH - 1; c >= 0; --c) {
+                const int k = s - 1 + c;
+                float y2k = 0.0f;
+                if (k >= 0 && k < nt - 1) y2k = y2f[k < 31 ? k : 31] * nxt + v[c];
+                v[c] = nxt;
+                nxt = y2k;
+            }
         }
-        __syncthreads();  // staging area free for the next tile
+        y2_carry = nxt;
+        if (MODE == 0) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) v[c] = h * v[c];
+        }
+        row_write(S, v);  // own row, read above by this lane only
+        __syncthreads();  // output tile complete
+        if (MODE == 0) {
+            // entries (y_k, h*y2_k) of rows 8q + lane/8, k = s + 4*(lane%8) .. +3:
+            // y from the register tile, h*y2 from LDS, 32 contiguous bytes per lane
+            const int cc = 4 * (lane & 7);
+            float *tab = static_cast<float *>(outp) + c0 * (int64_t)(2 * (nt + 1));
+            const uint32_t o = (uint32_t)(2 * ((lane >> 3) * (nt + 1) + s + cc));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int r = 8 * q + (lane >> 3);
+                if (r >= nrow) continue;
+                const float4 ov = *reinterpret_cast<const float4 *>(&S[r][cc]);
+                const F4u &y = py.q[q];
+                float *dst = at(tab, o + (uint32_t)(16 * q * (nt + 1)));
+                if (s + cc + 4 <= nt) {
+                    F4u w0{{y.v[0], ov.x, y.v[1], ov.y}};
+                    F4u w1{{y.v[2], ov.z, y.v[3], ov.w}};
+                    reinterpret_cast<F4u *>(dst)[0] = w0;
+                    reinterpret_cast<F4u *>(dst)[1] = w1;
+                } else {
+                    const float oo[4] = {ov.x, ov.y, ov.z, ov.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (s + cc + e < nt) {
+                            dst[2 * e] = y.v[e];
+                            dst[2 * e + 1] = oo[e];
+                        }
+                }
+            }
+            if (cb > 0) tile_fetch(ybase, nt, nrow, s - CH, nt, py);
+        } else {
+            float *y2 = static_cast<float *>(outp) + c0 * (int64_t)nt;
+            lds_to_global(S, y2, nt, nrow, s, nt);
+        }
+        __syncthreads();  // staging tile free for the next tile
     }
 }
 
@@ -268,6 +337,7 @@ int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, floa
                         float2 *coef, hipStream_t s)
 {
     if (nch <= 0 || nt <= 0) return 0;
+    if (nt > SPL_MAX_NT) return fail(1, "spline: nt %d exceeds %d samples", nt, SPL_MAX_NT);
     dim3 grid((unsigned)((nch + 63) / 64));
     ProfScope prof("spline_coefs", s);
     hipLaunchKernelGGL(k_spline<0>, grid, dim3(64), 0, s, rf, nch, (int)nt, dt, uscr,
@@ -279,6 +349,7 @@ int launch_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *
                      hipStream_t s)
 {
     if (nch <= 0 || nt <= 0) return 0;
+    if (nt > SPL_MAX_NT) return fail(1, "spline: nt %d exceeds %d samples", nt, SPL_MAX_NT);
     dim3 grid((unsigned)((nch + 63) / 64));
     hipLaunchKernelGGL(k_spline<1>, grid, dim3(64), 0, s, rf, nch, (int)nt, dt, uscr, (void *)y2);
     return check_launch("spline y2");

@@ -74,15 +74,21 @@ def test_index_selection_bitwise(c0, kind, usf):
     assert (ref >= 0).mean() > 0.5
 
 
-@pytest.mark.parametrize("nt", [1, 2, 3, 31, 33, 64, 100, 2048])
-def test_spline_prefilter_bitwise(nt):
+@pytest.mark.parametrize("data", ["normal", "sparse"])
+@pytest.mark.parametrize("nt", [1, 2, 3, 31, 32, 33, 64, 96, 100, 2048, 4096])
+def test_spline_prefilter_bitwise(nt, data):
     """Second derivatives (pycbf_spline_y2) and the gather table
-    (pycbf_spline_coefs) are bit-identical to tie_knots1D_fixed's."""
+    (pycbf_spline_coefs) are bit-identical to tie_knots1D_fixed's, for dense
+    data and for sparse spikes between exact zeros (whose recurrence tails
+    decay through the subnormal range)."""
     import torch
     from pycbf_amd.gpu import _engine as E
+    print(f"spline nt={nt} data={data}", flush=True)
     rng = np.random.default_rng(nt)
     nch, dt = 70, np.float32(2.5e-8)
     rf = rng.standard_normal((nch, nt)).astype(np.float32)
+    if data == "sparse":
+        rf *= (rng.random((nch, nt)) < 0.01).astype(np.float32)
     dev = torch.device("cuda")
     x = torch.as_tensor(rf, device=dev)
     wsb = E.lib.pycbf_spline_workspace_bytes(nch, nt)
