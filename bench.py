@@ -162,7 +162,7 @@ def slsc_leg(dev, reps, with_cpu):
                          SLSC_SUB, device=dev)
     out = torch.empty((1, SLSC_NXO, SLSC_NZO), dtype=torch.float32, device=dev)
     names = {"fp32": "slsc_fp32", "fp16": "slsc_mfma_fp16", "bf16": "slsc_mfma_bf16"}
-    ms = {}
+    ms, outs = {}, {}
     for prec, kname in names.items():
         slsc(im4, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, None, 1, SLSC_SUB, prec, out=out, sel=sel)
         torch.cuda.synchronize()
@@ -174,6 +174,7 @@ def slsc_leg(dev, reps, with_cpu):
         E.lib.pycbf_profile_enable(0)
         k_ms, k_n = E.profile_read(kname)
         ms[prec] = k_ms / max(k_n, 1)
+        outs[prec] = out[0].clone()
         log(f"slsc {prec}: {ms[prec] * 1e3:.1f} us/launch")
     npx = SLSC_NXO * SLSC_NZO
     algo = 4.0 * NRX * nx * nzf + 36.0 * npx     # per-rx images + selection bits + output
@@ -196,9 +197,19 @@ def slsc_leg(dev, reps, with_cpu):
         sub = imrx.view(NRX, nx, nzf)[:, cols].cpu().numpy()
         asub = bf.tables()[3].view(NRX, nx, nzf)[:, cols].cpu().numpy()
         t0 = time.perf_counter()
-        numerics.slsc_image(sub, asub, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, 1, SLSC_SUB)
+        ref = numerics.slsc_image(sub, asub, SLSC_LAGS, SLSC_KWIN, -SLSC_KWIN // 2, 1, SLSC_SUB)
         wall = time.perf_counter() - t0
         n = len(cols) * SLSC_NZO
+        # error of each precision on the sampled columns (coherence in [-1, 1];
+        # NaN where the oracle has NaN counts as agreement)
+        err = {}
+        for prec, o in outs.items():
+            g = o[cols].double().cpu().numpy()
+            both = np.isnan(g) & np.isnan(ref)
+            d = np.where(both, 0.0, np.abs(g - ref))
+            err[prec] = float(np.nanmax(np.where(np.isnan(d), np.inf, d)))
+        res["max_abs_err_vs_fp64"] = {p: float(f"{e:.3g}") for p, e in err.items()}
+        log(f"slsc max abs error vs FP64 on {n} px: {err}")
         res["cpu_baseline"] = {"value": round(n / wall, 1), "unit": "px/s", "cores": 1,
                                "kind": "port",
                                "sample": f"notebook pixel loop (numpy FP64) on {len(cols)} of "

@@ -28,8 +28,10 @@
 // leaves G[i][j] of a block in lane (j + 32*h), register r, with
 // i = 8*(r/4) + 4*h + r%4.  Taking G0[i][j] if j >= i else G1[i][j] and
 // rotating each register row left by i lanes (ds_bpermute) lands every entry
-// of lag 32q + m' in lane m', so one add per register accumulates all lags of
-// a block pair at once; no per-pair value leaves the registers.
+// of lag 32q + m' in lane m', so one add per register accumulates all lags at
+// once; no per-pair value leaves the registers.  The extraction is linear and
+// the same for every block, so the blocks (b, b+q) and (b, b+q+1) of all b are
+// first summed in the MFMA accumulators and extracted once per lag group.
 #include <math.h>
 #include <stdlib.h>
 
@@ -81,9 +83,7 @@ struct SlscArgs {
     float *out;         // (nfr, nxo, nzo)
     int64_t nrx, nx, nz;
     int nxo, nzo, kwin, kofs, dsfx, dsfz;
-    int pz;        // output pixels per workgroup (one axial strip)
-    int span;      // staged samples per channel
-    int xpitch;    // floats per staged channel row
+    int pz;        // consecutive output pixels per wave
     int zbytes;    // bytes of one wave's normalised-channel buffer
     int wbytes;    // bytes of one wave's LDS region (channels, flags, reduction rows)
     int nlag, need, ngroups;
@@ -91,84 +91,86 @@ struct SlscArgs {
     int lags[64];
 };
 
-template <int PREC, int KP>
+// 16 bytes at 4-byte alignment (gfx950 serves unaligned dwordx4 loads)
+struct __attribute__((aligned(4))) F4s {
+    float v[4];
+};
+
+// One wave per run of pz consecutive output pixels of a column; no
+// workgroup-level staging or barriers.  Lane l owns channels l, l+64, ...
+// (NG groups of 64) and loads each channel's window straight from the image:
+// 32 bytes per lane for K = 8, so the windows of consecutive pixels of a
+// wave share cache lines, which L2 serves after the first pixel.
+template <int PREC, int KP, int NG>
 __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nrx = (int)a.nrx;
-    float *xs = reinterpret_cast<float *>(smem);
-    const size_t zoff = align_up((size_t)nrx * a.xpitch * 4, 16);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    unsigned char *zbuf = smem + zoff + (size_t)wave * a.wbytes;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
+    unsigned char *zbuf = smem + (size_t)wave * a.wbytes;
     unsigned char *vflag = zbuf + a.zbytes;
     float *red = reinterpret_cast<float *>(vflag + 256);
 
     const int f = blockIdx.z, ixo = blockIdx.y;
     const int ix = ixo * a.dsfx;
-    const int izo0 = blockIdx.x * a.pz;
-    const int zs0 = izo0 * a.dsfz + a.kofs;
     const int64_t rstride = a.nx * a.nz;
+    const float *col = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;  // + r * rstride + z
+    const int izo0 = (blockIdx.x * nwave + wave) * a.pz;
 
-    // stage the strip's samples of every channel (zero outside the image): a
-    // wave takes every nwave-th channel, lanes run along z, and the loads of
-    // SB channels are in flight before any is stored
-    {
-        constexpr int SB = 16;
-        const float *src = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;
-        for (int zc = 0; zc < a.span; zc += 64) {
-            const int zz = zc + lane, z = zs0 + zz;
-            const bool in = zz < a.span && z >= 0 && (int64_t)z < a.nz;
-            for (int r0 = wave; r0 < nrx; r0 += nwave * SB) {
-                float v[SB];
-#pragma unroll
-                for (int i = 0; i < SB; ++i) {
-                    const int r = r0 + i * nwave;
-                    v[i] = (in && r < nrx) ? src[r * rstride + z] : 0.0f;
-                }
-#pragma unroll
-                for (int i = 0; i < SB; ++i) {
-                    const int r = r0 + i * nwave;
-                    if (r < nrx && zz < a.span) xs[r * a.xpitch + zz] = v[i];
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    for (int k = wave; k < a.pz; k += nwave) {
+    for (int k = 0; k < a.pz; ++k) {
         const int izo = izo0 + k;
         if (izo >= a.nzo) break;
         const int iz = izo * a.dsfz;
         const int w0 = max(iz + a.kofs, 0);
         const int w1 = min((int64_t)(iz + a.kofs + a.kwin), a.nz);
-        const int keff = w1 - w0, off = w0 - zs0;
+        const int keff = w1 - w0;
         const float rk = 1.0f / (float)keff;
         // |std| >= 1e-10  <=>  sum of squared deviations >= 1e-20 * K
         const float vmin = 1e-20f * (float)keff;
 
-        // select, normalise and compact the channels
+        // windows of every usable channel, all loads in flight before use
         const unsigned long long *pbits =
             a.sel ? a.sel + ((int64_t)ixo * a.nzo + izo) * 4 : nullptr;
-        int n = 0;
-        for (int c = 0; c < nrx; c += 64) {
-            const int r = c + lane;
-            bool sel = r < nrx;
-            if (pbits) sel = sel && ((pbits[c >> 6] >> lane) & 1ull);
-            const float *xr = xs + (sel ? r : 0) * a.xpitch + off;
-            float xv[KP];
+        float xv[NG][KP];
+        bool selg[NG];
 #pragma unroll
-            for (int w = 0; w < KP; ++w) xv[w] = (sel && w < keff) ? xr[w] : 0.0f;
+        for (int g = 0; g < NG; ++g) {
+            const int r = 64 * g + lane;
+            bool sel = r < nrx;
+            if (pbits) sel = sel && ((pbits[g] >> lane) & 1ull);
+            selg[g] = sel;
+            const float *xr = col + (int64_t)(sel ? r : 0) * rstride + w0;
+            if (keff == KP) {
+#pragma unroll
+                for (int w = 0; w < KP; w += 4) {
+                    F4s v{{0.0f, 0.0f, 0.0f, 0.0f}};
+                    if (sel) v = *reinterpret_cast<const F4s *>(xr + w);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) xv[g][w + e] = v.v[e];
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < KP; ++w) xv[g][w] = (sel && w < keff) ? xr[w] : 0.0f;
+            }
+        }
+
+        // normalise and compact the usable channels
+        int n = 0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const bool sel = selg[g];
             float mu = 0.0f, rs = 0.0f;
             bool valid = false;
             if (sel) {
                 float s = 0.0f;
 #pragma unroll
-                for (int w = 0; w < KP; ++w) s += xv[w];
+                for (int w = 0; w < KP; ++w) s += xv[g][w];
                 mu = s * rk;
                 float v = 0.0f;
 #pragma unroll
                 for (int w = 0; w < KP; ++w) {
-                    const float d = w < keff ? xv[w] - mu : 0.0f;
+                    const float d = w < keff ? xv[g][w] - mu : 0.0f;
                     v = __builtin_fmaf(d, d, v);
                 }
                 // z = (x - mean) / (std * sqrt(K)) = (x - mean) / sqrt(sum d^2)
@@ -182,13 +184,13 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
                 if (PREC == 0) {
                     float *zr = reinterpret_cast<float *>(zbuf) + p * (KP + 1);
 #pragma unroll
-                    for (int w = 0; w < KP; ++w) zr[w] = w < keff ? (xv[w] - mu) * rs : 0.0f;
+                    for (int w = 0; w < KP; ++w) zr[w] = w < keff ? (xv[g][w] - mu) * rs : 0.0f;
                 } else {
                     unsigned *zr = reinterpret_cast<unsigned *>(zbuf) + p * (KP / 2);
                     unsigned hv[KP];
 #pragma unroll
                     for (int w = 0; w < KP; ++w) {
-                        const float z = w < keff ? (xv[w] - mu) * rs : 0.0f;
+                        const float z = w < keff ? (xv[g][w] - mu) * rs : 0.0f;
                         hv[w] = PREC == 1 ? (unsigned)__builtin_bit_cast(unsigned short, (_Float16)z)
                                           : (unsigned)to_bf16(z);
                     }
@@ -263,9 +265,11 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
             const int j = lane & 31, h = lane >> 5;
             const int mylag = lane < a.nlag ? a.lags[lane] : 0;
             for (int q = 0; q < a.ngroups; ++q) {
-                float acc = 0.0f;
+                // The band extraction below is linear and the same for every
+                // block pair, so the Gram blocks (b, b+q) and (b, b+q+1) of all
+                // b are summed in the MFMA accumulators and extracted once.
+                f32x16_t g0 = {}, g1 = {};
                 for (int b = 0; b + q < nblk; ++b) {
-                    f32x16_t g0 = {}, g1 = {};
                     const bool two = b + q + 1 < nblk;
 #pragma unroll
                     for (int kc = 0; kc < KP / 8; ++kc) {
@@ -289,14 +293,15 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
                                     va, *reinterpret_cast<const bf16x4_t *>(pb1), g1, 0, 0, 0);
                         }
                     }
+                }
+                float acc = 0.0f;
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-                        const float v = j >= i ? g0[r] : g1[r];
-                        const int srcl = 32 * h + ((j + i) & 31);
-                        acc += __int_as_float(
-                            __builtin_amdgcn_ds_bpermute(srcl << 2, __float_as_int(v)));
-                    }
+                for (int r = 0; r < 16; ++r) {
+                    const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+                    const float v = j >= i ? g0[r] : g1[r];
+                    const int srcl = 32 * h + ((j + i) & 31);
+                    acc += __int_as_float(
+                        __builtin_amdgcn_ds_bpermute(srcl << 2, __float_as_int(v)));
                 }
                 acc += __shfl_xor(acc, 32);
                 const float got = __shfl(acc, mylag & 31);
@@ -353,45 +358,31 @@ int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t 
     return check_launch("slsc_selection");
 }
 
-// Staging geometry and LDS size for a launch; returns 0 or an error status.
-static int slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
+// Per-wave LDS (normalised channels, flags, FP32 reduction rows), waves per
+// workgroup and pixels per wave of a launch.
+static void slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
 {
     const int nrx = (int)a.nrx;
     const int npad = (nrx + 31) / 32 * 32;
     a.zbytes = (int)align_up(prec == 0 ? (size_t)npad * (kp + 1) * 4 : (size_t)npad * kp * 2, 16);
-    const size_t budget = 64 * 1024;
     a.wbytes = (int)align_up((size_t)a.zbytes + 256 + (prec == 0 ? 8 * 65 * 4 : 0), 16);
-    // strip length: keep the staged channels within about 32 KB (PYCBF_SLSC_PZ
-    // overrides it, still bounded by the LDS budget below)
-    int pz = 16;
-    for (; pz > 1; --pz) {
-        const int span = (pz - 1) * a.dsfz + a.kwin;
-        if ((size_t)nrx * (span + 1) * 4 <= 32 * 1024) break;
-    }
+    // 2 pixels per wave: more waves in flight beat longer runs per wave
+    // (measured on 128 channels, K = 8, dsfz = 8: 2 px 76 us, 4 px 92 us, 8 px
+    // 154 us for the FP16 kernel)
+    int pz = 2;
     if (const char *e = getenv("PYCBF_SLSC_PZ")) pz = std::max(1, std::min(64, atoi(e)));
-    pz = std::min<int>(pz, a.nzo);
-    for (;;) {
-        a.pz = pz;
-        a.span = (pz - 1) * a.dsfz + a.kwin;
-        a.xpitch = a.span | 1;
-        const size_t stage = align_up((size_t)nrx * a.xpitch * 4, 16);
-        for (nwave = std::min(4, pz); nwave >= 1; --nwave) {
-            lds = stage + (size_t)nwave * a.wbytes;
-            if (lds <= budget) return 0;
-        }
-        if (pz == 1) break;
-        pz = std::max(1, pz / 2);
-    }
-    return fail(PYCBF_EINVAL, "SLSC of %d channels with a %d-sample window does not fit in LDS",
-                nrx, a.kwin);
+    a.pz = pz;
+    nwave = 4;
+    lds = (size_t)nwave * a.wbytes;
 }
 
-template <int PREC, int KP>
+template <int PREC, int KP, int NG>
 static int launch_slsc_t(const SlscArgs &a, int nwave, size_t lds, int64_t nfr, hipStream_t s)
 {
-    const dim3 grid((unsigned)((a.nzo + a.pz - 1) / a.pz), (unsigned)a.nxo, (unsigned)nfr);
+    const int per_block = a.pz * nwave;
+    const dim3 grid((unsigned)((a.nzo + per_block - 1) / per_block), (unsigned)a.nxo, (unsigned)nfr);
     ProfScope ps(PREC == 0 ? "slsc_fp32" : (PREC == 1 ? "slsc_mfma_fp16" : "slsc_mfma_bf16"), s);
-    hipLaunchKernelGGL((k_slsc<PREC, KP>), grid, dim3(64 * nwave), lds, s, a);
+    hipLaunchKernelGGL((k_slsc<PREC, KP, NG>), grid, dim3(64 * nwave), lds, s, a);
     return check_launch("slsc");
 }
 
@@ -399,10 +390,17 @@ template <int PREC>
 static int launch_slsc_p(const SlscArgs &a, int kp, int nwave, size_t lds, int64_t nfr,
                          hipStream_t s)
 {
+    if (a.nrx <= 128) {
+        switch (kp) {
+        case 8: return launch_slsc_t<PREC, 8, 2>(a, nwave, lds, nfr, s);
+        case 16: return launch_slsc_t<PREC, 16, 2>(a, nwave, lds, nfr, s);
+        default: return launch_slsc_t<PREC, 32, 2>(a, nwave, lds, nfr, s);
+        }
+    }
     switch (kp) {
-    case 8: return launch_slsc_t<PREC, 8>(a, nwave, lds, nfr, s);
-    case 16: return launch_slsc_t<PREC, 16>(a, nwave, lds, nfr, s);
-    default: return launch_slsc_t<PREC, 32>(a, nwave, lds, nfr, s);
+    case 8: return launch_slsc_t<PREC, 8, 4>(a, nwave, lds, nfr, s);
+    case 16: return launch_slsc_t<PREC, 16, 4>(a, nwave, lds, nfr, s);
+    default: return launch_slsc_t<PREC, 32, 4>(a, nwave, lds, nfr, s);
     }
 }
 
@@ -435,7 +433,7 @@ int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t n
     const int kp = kwin <= 8 ? 8 : (kwin <= 16 ? 16 : 32);
     int nwave = 0;
     size_t lds = 0;
-    if (int rc = slsc_plan(a, precision, kp, nwave, lds)) return rc;
+    slsc_plan(a, precision, kp, nwave, lds);
     switch (precision) {
     case PYCBF_SLSC_FP32: return launch_slsc_p<0>(a, kp, nwave, lds, nfr, s);
     case PYCBF_SLSC_MFMA_FP16: return launch_slsc_p<1>(a, kp, nwave, lds, nfr, s);

@@ -43,6 +43,8 @@ CASES = {
     "200rx_k5_lags1-40_apod_ds": (200, 6, 31, 5, -2, 2, 3, list(range(1, 41)), True),
     "256rx_k17_sparse_lags": (256, 2, 24, 17, -8, 1, 2, [1, 3, 7, 31, 32, 33, 63], False),
     "40rx_k3_few_elements": (40, 3, 20, 3, -1, 1, 1, [1, 2, 5], True),
+    "96rx_k16_lags1-20": (96, 3, 48, 16, -8, 1, 4, list(range(1, 21)), False),
+    "160rx_k32_apod": (160, 2, 70, 32, -16, 1, 8, [1, 2, 4, 8, 16], True),
 }
 
 
