@@ -291,19 +291,29 @@ static int upload(BatchPlan &P, const BatchPtrs &b, hipStream_t s)
     return check(hipStreamSynchronize(s), "batch upload");
 }
 
-static int persistent_blocks()
+// Workgroups of a persistent launch: as many as can be resident at once
+// (every CU times the occupancy the kernel's registers allow), so that the
+// queue is drained by the full complement of waves.
+template <typename K>
+static int persistent_blocks(K kernel)
 {
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    return cus * 4;
+    int dev = 0, cus = 256, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 4;
+    return cus * per_cu;
 }
 
 template <int TILE, int RC>
 static void launch_gather_t(const BatchPlan &P, const BatchPtrs &b, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_batch_das<TILE, RC>), dim3(persistent_blocks()), dim3(256), 0, s, b.reqs,
-                       b.items, P.nitems, b.counter, b.finish);
+    static const int blocks = persistent_blocks(k_batch_das<TILE, RC>);
+    hipLaunchKernelGGL((k_batch_das<TILE, RC>), dim3(blocks), dim3(256), 0, s, b.reqs, b.items,
+                       P.nitems, b.counter, b.finish);
 }
 
 static int launch_gather(const BatchPlan &P, const BatchPtrs &b, hipStream_t s)

@@ -254,6 +254,7 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
         break;
     }
     }
+    if (das_reg_applies(plan.gk, nfr, sumflag)) return launch_das_reg(a, s);
     float *part = nullptr;
     if (plan.part) {
         part = (float *)cv.take(plan.part);

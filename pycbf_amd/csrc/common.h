@@ -89,6 +89,10 @@ int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t
 // LDS-staged gather (delay tables or in-register synthetic geometry); bounds
 // must hold the tile delay ranges written by launch_tile_bounds.
 int launch_das_lds(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
+// Single-frame natural-cubic gather with transmit delays in registers: used
+// for one frame summed over tx and rx, or over tx only (das_reg.hip).
+bool das_reg_applies(int gk, int64_t nfr, int sumflag);
+int launch_das_reg(const DasArgs &a, hipStream_t s);
 // Delay min/max per (pixel tile of 256, element): tx elements then rx.
 int launch_tile_bounds(const DasArgs &a, float *bounds, hipStream_t s);
 inline int64_t das_tiles(int64_t np) { return (np + 255) / 256; }

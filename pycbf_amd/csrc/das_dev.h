@@ -124,6 +124,22 @@ __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, 
 // ---------------------------------------------------------------------------
 // gather + interpolate for FC frames of one channel
 // ---------------------------------------------------------------------------
+// delay and weight of element e of the tx (or rx) axis at pixel p
+template <int GEO>
+__device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int64_t p,
+                                          const float (&pv)[3], float &tau, float &apod)
+{
+    if (GEO == 0) {
+        const int64_t o = (int64_t)e * A.np + p;
+        tau = tx ? A.ttx[o] : A.trx[o];
+        apod = tx ? A.atx[o] : A.arx[o];
+    } else if (tx) {
+        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
+    } else {
+        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
+    }
+}
+
 template <int GK, int FC>
 __device__ __forceinline__ void gather(const DasArgs &A, int64_t ch0, int64_t fstride, int nf,
                                        float x, bool &ok, float (&v)[FC])

@@ -49,22 +49,6 @@ struct Stage {
     static constexpr int NLD = (WIN + TPF - 1) / TPF;
 };
 
-// delay and weight of element e of the tx (or rx) axis at pixel p
-template <int GEO>
-__device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int64_t p,
-                                          const float (&pv)[3], float &tau, float &apod)
-{
-    if (GEO == 0) {
-        const int64_t o = (int64_t)e * A.np + p;
-        tau = tx ? A.ttx[o] : A.trx[o];
-        apod = tx ? A.atx[o] : A.arx[o];
-    } else if (tx) {
-        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
-    } else {
-        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // delay range per (tile, element) over the pixels that can contribute
 // ---------------------------------------------------------------------------
