@@ -201,18 +201,22 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
  *   from q = (x - z tan(steer), 0) to the pixel;
  * receive elements at (xele, 0) (device (nrx)): taurx = integral of slow from
  * the element to the pixel.  Midpoint rule with nsteps samples per ray.
- * pvec (np, 2) device pixel positions (x, z); tautx (ntx, np), taurx (nrx, np). */
+ * pvec (np, 2) device pixel positions (x, z); tautx (ntx, np), taurx (nrx, np).
+ * The map has at most 8192 nodes. */
 int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                      float dzm, const float *steer, const float *xorig, int ntx, float c0,
                      const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
                      float *tautx, float *taurx, void *stream);
 
-/* Workspace bytes of pycbf_ray_delays_bwd (partial maps of its workgroups). */
+/* Workspace bytes of pycbf_ray_delays_bwd (64-bit partial maps of its
+ * workgroups). */
 size_t pycbf_ray_delays_bwd_workspace_bytes(int nzm, int nxm);
 
 /* Adjoint of pycbf_ray_delays: gslow (nzm, nxm) = d loss / d slow given
  * gtautx (ntx, np) and gtaurx (nrx, np) (either may be NULL).  gslow is
- * written. */
+ * written.  The map sums are accumulated in 64-bit fixed point (scaled by the
+ * largest |g| * path length), so the result does not depend on scheduling;
+ * a non-finite upstream gradient gives NaN in every node. */
 int pycbf_ray_delays_bwd(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                          float dzm, const float *steer, const float *xorig, int ntx, float c0,
                          const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,

@@ -150,7 +150,8 @@ struct RayArgs {
     float *tautx, *taurx;              // forward outputs
     const float *gtautx, *gtaurx;      // adjoint inputs
     float *gslow;                      // adjoint output
-    float *gpart;                      // adjoint scratch: partial maps of the workgroups
+    float *gpart;                      // adjoint scratch: 256 B (max word), then the workgroups'
+                                       // 64-bit fixed-point partial maps
 };
 
 int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s);

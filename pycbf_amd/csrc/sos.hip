@@ -24,7 +24,7 @@ namespace pycbf {
 
 namespace {
 
-constexpr int SOS_MAXMAP = 16384;   // map nodes held in LDS (64 KB)
+constexpr int SOS_MAXMAP = 8192;    // map nodes held in LDS (64 KB of 64-bit sums in the adjoint)
 constexpr int SOS_PARTIALS = 1024;  // workgroups (partial maps) of the adjoint
 
 // bilinear sample positions: node indices and weights of point (x, z)
@@ -62,6 +62,11 @@ __device__ __forceinline__ void ray_of(int e, int ntx, const float *steer, const
 
 }  // namespace
 
+__global__ void k_fill_u32(unsigned *p, int n, unsigned v)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
 // one lane per (element, pixel); the map is staged in LDS
 __global__ __launch_bounds__(256) void k_ray_delays(RayArgs a)
 {
@@ -96,30 +101,81 @@ __global__ __launch_bounds__(256) void k_ray_delays(RayArgs a)
     else a.taurx[(int64_t)(e - a.ntx) * a.np + p] = tau;
 }
 
+// Adjoint accumulation in 64-bit fixed point.  Float atomic adds in LDS run
+// at about 0.3 lanes per clock per CU on gfx950, integer adds at about 12, so
+// the map sums are integers: a contribution c is added as round(c * scale).
+// k_ray_scale finds M = max over rays of |g| * L, which bounds what one ray
+// adds to any node (its bilinear weights sum to at most nsteps per node and
+// gl = g L / nsteps), so |sum over all rays| <= total * M, and
+// scale = 2^62 / (total * M) keeps every partial and final sum within int64.
+// The rounding error per contribution is at most 2^-63 * total * M, and
+// integer sums do not depend on the order of the atomics: the map gradient
+// is deterministic.
+__global__ __launch_bounds__(256) void k_ray_scale(RayArgs a, unsigned *mbits)
+{
+    const int64_t total = (int64_t)(a.ntx + a.nrx) * a.np;
+    float m = 0.0f;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < total;
+         q += (int64_t)gridDim.x * 256) {
+        const int e = (int)(q / a.np);
+        const int64_t p = q - (int64_t)e * a.np;
+        const float g = e < a.ntx ? (a.gtautx ? a.gtautx[q] : 0.0f)
+                                  : (a.gtaurx ? a.gtaurx[q - (int64_t)a.ntx * a.np] : 0.0f);
+        if (g == 0.0f) continue;
+        float qx, L, fire;
+        ray_of(e, a.ntx, a.steer, a.xorig, a.xele, a.c0, a.pvec[2 * p], a.pvec[2 * p + 1], qx, L,
+               fire);
+        const float v = fabsf(g * L);
+        // NaN or infinity: the largest pattern, read back as "not finite"
+        m = (v <= 3.0e38f) ? fmaxf(m, v) : __int_as_float(0x7f800000);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    // non-negative floats order like their bit patterns
+    if ((threadIdx.x & 63) == 0) atomicMax(mbits, __float_as_uint(m));
+}
+
+__device__ __forceinline__ double ray_scale(const RayArgs &a, const unsigned *mbits)
+{
+    const float m = __uint_as_float(*mbits);
+    const double total = (double)(a.ntx + a.nrx) * (double)a.np;
+    return (m > 0.0f && m <= 3.0e38f) ? 4611686018427387904.0 / (total * (double)m) : 0.0;
+}
+
 // adjoint: one lane per ray.  Rays are visited in a scrambled order
-// (q -> q * stride mod total, stride coprime to total) so that the lanes of a
-// wave follow unrelated rays and rarely touch the same map cell at the same
-// step; along its ray a lane accumulates the bilinear weights of the current
-// cell in registers and adds them to the LDS map only when the ray leaves the
-// cell.  Each workgroup stores its LDS map as a partial map, and k_ray_reduce
-// sums the partials (no device-wide atomics on the few map nodes).
-__device__ __forceinline__ void flush_cell(float *gm, int nxm, int nzm, int i0, int k0, float gl,
-                                           float a00, float a01, float a10, float a11)
+// (q -> q * stride mod total, stride coprime to total) and each lane starts
+// its ray at a different step, so that the lanes of a wave follow unrelated
+// rays at different depths; along its ray a lane accumulates the bilinear
+// weights of the current cell in registers and adds them to the LDS map only
+// when the ray leaves the cell.  Each workgroup stores its LDS map as a
+// partial map, and k_ray_reduce sums the partials (no device-wide atomics on
+// the few map nodes).
+__device__ __forceinline__ void flush_cell(unsigned long long *gm, int nxm, int nzm, int i0, int k0,
+                                           float gl, double scale, float a00, float a01,
+                                           float a10, float a11)
 {
     if (i0 < 0) return;
     const int i1 = min(i0 + 1, nzm - 1), k1 = min(k0 + 1, nxm - 1);
-    atomicAdd(&gm[i0 * nxm + k0], gl * a00);
-    atomicAdd(&gm[i0 * nxm + k1], gl * a01);
-    atomicAdd(&gm[i1 * nxm + k0], gl * a10);
-    atomicAdd(&gm[i1 * nxm + k1], gl * a11);
+    // weights that stayed zero over the run (clamped edges) add nothing
+    auto add = [&](int idx, float w) {
+        if (w != 0.0f)
+            atomicAdd(&gm[idx], (unsigned long long)__double2ll_rn((double)(gl * w) * scale));
+    };
+    add(i0 * nxm + k0, a00);
+    add(i0 * nxm + k1, a01);
+    add(i1 * nxm + k0, a10);
+    add(i1 * nxm + k1, a11);
 }
 
-__global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, int64_t stride, int64_t step)
+__global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigned *mbits,
+                                                        long long *gpart, int64_t stride,
+                                                        int64_t step)
 {
-    extern __shared__ float gm[];
+    extern __shared__ unsigned long long gm64[];
     const int nmap = a.g.nzm * a.g.nxm;
-    for (int t = threadIdx.x; t < nmap; t += 256) gm[t] = 0.0f;
+    for (int t = threadIdx.x; t < nmap; t += 256) gm64[t] = 0ull;
     __syncthreads();
+    const double scale = ray_scale(a, mbits);
     const int64_t total = (int64_t)(a.ntx + a.nrx) * a.np;
     const float rn = 1.0f / (float)a.nsteps;
     // ray r -> q = r * stride mod total; successive r of a lane differ by
@@ -127,80 +183,108 @@ __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, int64_t strid
     const int64_t r0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int64_t q = (int64_t)(((unsigned __int128)r0 * (unsigned __int128)stride) %
                           (unsigned __int128)total);
-    for (int64_t r = r0; r < total; r += (int64_t)gridDim.x * 256,
-                 q = (q + step >= total) ? q + step - total : q + step) {
-        const int e = (int)(q / a.np);
-        const int64_t p = q - (int64_t)e * a.np;
-        const float g = e < a.ntx ? (a.gtautx ? a.gtautx[q] : 0.0f)
-                                  : (a.gtaurx ? a.gtaurx[q - (int64_t)a.ntx * a.np] : 0.0f);
-        if (g == 0.0f) continue;
-        const float px = a.pvec[2 * p], pz = a.pvec[2 * p + 1];
-        float qx, L, fire;
-        ray_of(e, a.ntx, a.steer, a.xorig, a.xele, a.c0, px, pz, qx, L, fire);
-        const float gl = g * L * rn;
-        int ci = -1, ck = -1;
-        float a00 = 0.0f, a01 = 0.0f, a10 = 0.0f, a11 = 0.0f;
-        for (int m = 0; m < a.nsteps; ++m) {
-            const float t = ((float)m + 0.5f) * rn;
-            const float x = qx + t * (px - qx), z = t * pz;
-            int i0, k0;
-            float wz, wx;
-            bilin(a.g, x, z, i0, k0, wz, wx);
-            if (i0 != ci || k0 != ck) {
-                flush_cell(gm, a.g.nxm, a.g.nzm, ci, ck, gl, a00, a01, a10, a11);
-                ci = i0;
-                ck = k0;
-                a00 = a01 = a10 = a11 = 0.0f;
+    if (scale > 0.0) {
+        for (int64_t r = r0; r < total; r += (int64_t)gridDim.x * 256,
+                     q = (q + step >= total) ? q + step - total : q + step) {
+            const int e = (int)(q / a.np);
+            const int64_t p = q - (int64_t)e * a.np;
+            const float g = e < a.ntx ? (a.gtautx ? a.gtautx[q] : 0.0f)
+                                      : (a.gtaurx ? a.gtaurx[q - (int64_t)a.ntx * a.np] : 0.0f);
+            if (g == 0.0f) continue;
+            const float px = a.pvec[2 * p], pz = a.pvec[2 * p + 1];
+            float qx, L, fire;
+            ray_of(e, a.ntx, a.steer, a.xorig, a.xele, a.c0, px, pz, qx, L, fire);
+            const float gl = g * L * rn;
+            int ci = -1, ck = -1;
+            float a00 = 0.0f, a01 = 0.0f, a10 = 0.0f, a11 = 0.0f;
+            // lane l walks its ray from step l (mod nsteps) on, so the lanes of
+            // a wave sit at different depths at any moment; the wrap costs one
+            // extra flush
+            const int mo = (int)(threadIdx.x & 63) % a.nsteps;
+            for (int mm = 0; mm < a.nsteps; ++mm) {
+                const int m = mm + mo < a.nsteps ? mm + mo : mm + mo - a.nsteps;
+                const float t = ((float)m + 0.5f) * rn;
+                const float x = qx + t * (px - qx), z = t * pz;
+                int i0, k0;
+                float wz, wx;
+                bilin(a.g, x, z, i0, k0, wz, wx);
+                if (i0 != ci || k0 != ck) {
+                    flush_cell(gm64, a.g.nxm, a.g.nzm, ci, ck, gl, scale, a00, a01, a10, a11);
+                    ci = i0;
+                    ck = k0;
+                    a00 = a01 = a10 = a11 = 0.0f;
+                }
+                const float uz = 1.0f - wz, ux = 1.0f - wx;
+                a00 += uz * ux;
+                a01 += uz * wx;
+                a10 += wz * ux;
+                a11 += wz * wx;
             }
-            const float uz = 1.0f - wz, ux = 1.0f - wx;
-            a00 += uz * ux;
-            a01 += uz * wx;
-            a10 += wz * ux;
-            a11 += wz * wx;
+            flush_cell(gm64, a.g.nxm, a.g.nzm, ci, ck, gl, scale, a00, a01, a10, a11);
         }
-        flush_cell(gm, a.g.nxm, a.g.nzm, ci, ck, gl, a00, a01, a10, a11);
     }
     __syncthreads();
-    float *part = a.gpart + (int64_t)blockIdx.x * nmap;
-    for (int t = threadIdx.x; t < nmap; t += 256) part[t] = gm[t];
+    long long *part = gpart + (int64_t)blockIdx.x * nmap;
+    for (int t = threadIdx.x; t < nmap; t += 256) part[t] = (long long)gm64[t];
 }
 
-// gslow[t] = sum over workgroups of their partial maps (fixed order)
-__global__ __launch_bounds__(256) void k_ray_reduce(const float *__restrict__ part, int nb, int nmap,
-                                                    float *__restrict__ gslow)
+// gslow[t] = (sum over workgroups of their partial maps) / scale; integer
+// sums are exact, so the result does not depend on the workgroup count or
+// order.  A non-finite upstream gradient gives NaN everywhere.
+__global__ __launch_bounds__(256) void k_ray_reduce(RayArgs a, const unsigned *mbits,
+                                                    const long long *__restrict__ part, int nb,
+                                                    int nmap, float *__restrict__ gslow)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= nmap) return;
-    float acc = 0.0f;
+    const float m = __uint_as_float(*mbits);
+    if (!(m <= 3.0e38f)) {
+        gslow[t] = __builtin_nanf("");
+        return;
+    }
+    const double scale = ray_scale(a, mbits);
+    if (scale == 0.0) {
+        gslow[t] = 0.0f;
+        return;
+    }
+    long long acc = 0;
     for (int b = 0; b < nb; ++b) acc += part[(int64_t)b * nmap + t];
-    gslow[t] = acc;
+    gslow[t] = (float)((double)acc / scale);
 }
 
 int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s)
 {
     const int nmap = a.g.nzm * a.g.nxm;
-    const size_t lds = (size_t)nmap * 4;
     if (!adjoint) {
         ProfScope ps("ray_delays", s);
         const dim3 grid((unsigned)((a.np + 255) / 256), (unsigned)(a.ntx + a.nrx));
-        hipLaunchKernelGGL(k_ray_delays, grid, dim3(256), lds, s, a);
+        hipLaunchKernelGGL(k_ray_delays, grid, dim3(256), (size_t)nmap * 4, s, a);
         return check_launch("ray_delays");
     }
     ProfScope ps("ray_delays_bwd", s);
     const int64_t total = (int64_t)(a.ntx + a.nrx) * a.np;
     const int nb = (int)std::min<int64_t>((total + 255) / 256, SOS_PARTIALS);
+    // workspace: the max word, then the partial maps (64-bit)
+    unsigned *mbits = reinterpret_cast<unsigned *>(a.gpart);
+    long long *part = reinterpret_cast<long long *>(reinterpret_cast<char *>(a.gpart) + 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, mbits, 1, 0u);
+    if (int rc = check_launch("ray_scale clear")) return rc;
+    const unsigned sb = (unsigned)std::min<int64_t>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_ray_scale, dim3(sb), dim3(256), 0, s, a, mbits);
+    if (int rc = check_launch("ray_scale")) return rc;
     // scramble stride: near total * 0.618, coprime to total
     int64_t stride = std::max<int64_t>(1, (int64_t)((double)total * 0.6180339887498949));
     while (std::__gcd(stride, total) != 1) ++stride;
     const int64_t step = (int64_t)(((unsigned __int128)((int64_t)nb * 256) * stride) % total);
-    hipLaunchKernelGGL(k_ray_delays_bwd, dim3((unsigned)nb), dim3(256), lds, s, a, stride, step);
+    hipLaunchKernelGGL(k_ray_delays_bwd, dim3((unsigned)nb), dim3(256), (size_t)nmap * 8, s, a,
+                       mbits, part, stride, step);
     if (int rc = check_launch("ray_delays_bwd")) return rc;
-    hipLaunchKernelGGL(k_ray_reduce, dim3((unsigned)((nmap + 255) / 256)), dim3(256), 0, s, a.gpart,
-                       nb, nmap, a.gslow);
+    hipLaunchKernelGGL(k_ray_reduce, dim3((unsigned)((nmap + 255) / 256)), dim3(256), 0, s, a,
+                       mbits, part, nb, nmap, a.gslow);
     return check_launch("ray_reduce");
 }
 
 int sos_max_map() { return SOS_MAXMAP; }
-size_t ray_bwd_workspace_bytes(int nzm, int nxm) { return (size_t)SOS_PARTIALS * nzm * nxm * 4; }
+size_t ray_bwd_workspace_bytes(int nzm, int nxm) { return 256 + (size_t)SOS_PARTIALS * nzm * nxm * 8; }
 
 }  // namespace pycbf
