@@ -30,6 +30,7 @@
 #include "common.h"
 #include "das_dev.h"
 #include "prof.h"
+#include "tiles.h"
 
 namespace pycbf {
 
@@ -252,40 +253,105 @@ __device__ __forceinline__ float elim_factor(int i)
     return q;
 }
 
-__global__ __launch_bounds__(256) void k_spline_adjoint(const float *__restrict__ gy,
-                                                        const float *__restrict__ ge, int gstride,
-                                                        int64_t nch, int n, float *__restrict__ drf)
+// One wave per 64 channels, one lane per channel for both recurrences; the
+// samples move in coalesced 64 x 32 tiles through one LDS tile (as in the
+// spline prefilter).  Tile s of the forward sweep computes d'_i for
+// i = s-1 .. s+30 (e[s-1] is carried), stored shifted as D[i+1] in drf, so
+// that tile s of the backward sweep reads only D tile s and writes the
+// outputs o[k], k = s .. s+31, over it in place:
+//   o[k] = y[k] + (u_{k-1} - 2 u_k + u_{k+1}),  u_j = 0 outside 1..n-2.
+__global__ __launch_bounds__(64) void k_spline_adjoint(const float *__restrict__ gy,
+                                                       const float *__restrict__ ge, int gstride,
+                                                       int64_t nch, int n, float *__restrict__ drf)
 {
-    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (c >= nch) return;
-    const float *y = gy + c * gstride;
-    const float *e = ge + c * gstride;
-    float *o = drf + c * (int64_t)n;
-    if (n < 3) {
-        for (int k = 0; k < n; ++k) o[k] = y[k];
-        return;
-    }
-    // forward elimination, d'_i kept in o[i] (i = 1 .. n-2)
-    float cp = 0.25f, dp = e[1] * 0.25f;
-    o[1] = dp;
-    for (int i = 2; i <= n - 2; ++i) {
-        cp = (i <= CP_EXACT) ? 1.0f / (4.0f - cp) : cp;
-        dp = (e[i] - dp) * cp;
-        o[i] = dp;
-    }
-    // back substitution u_i = d'_i - c'_i u_{i+1}; entry k = i + 1 is final
-    // once u_i is known (u_{n-1} = u_n = 0)
+    __shared__ __align__(16) SplRow S[64];
+    __shared__ float cpv[32];  // c'_i for i < 32 (c'_i = c'_CP_EXACT beyond)
+    const int lane = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int nrow = (int)((nch - c0) < 64 ? (nch - c0) : 64);
+    const float *ybase = gy + c0 * (int64_t)gstride;
+    const float *ebase = ge + c0 * (int64_t)gstride;
+    float *obase = drf + c0 * (int64_t)n;
+    if (lane < 32) cpv[lane] = elim_factor(lane < 1 ? 1 : lane);
     const float cinf = elim_factor(CP_EXACT);
-    float u1 = 0.0f, u2 = 0.0f;  // u_{i+1}, u_{i+2}
-    for (int i = n - 2; i >= 1; --i) {
-        const float ci = (i <= CP_EXACT) ? elim_factor(i) : cinf;
-        const float ui = o[i] - ci * u1;
-        o[i + 1] = y[i + 1] + (ui - 2.0f * u1 + u2);
-        u2 = u1;
-        u1 = ui;
+    const int nchunk = (n + 31) / 32;
+
+    // ---- forward elimination: d'_i = (e_i - d'_{i-1}) c'_i, d'_0 = 0 ----
+    TileRegs pf;
+    tile_fetch(ebase, gstride, nrow, 0, n, pf);
+    tile_to_lds(pf, S);
+    if (nchunk > 1) tile_fetch(ebase, gstride, nrow, 32, n, pf);
+    __syncthreads();
+    float e_prev = 0.0f, dp = 0.0f;
+    for (int cb = 0; cb < nchunk; ++cb) {
+        const int s = cb * 32;
+        float v[32];
+        row_read(S, v);
+        float ecar = e_prev;  // e_{s-1+c}
+#pragma unroll
+        for (int c = 0; c < 32; ++c) {
+            const int i = s - 1 + c;
+            const float ei = ecar;
+            ecar = v[c];  // e_{s+c}, read before v[c] is overwritten
+            float d = 0.0f;
+            if (i >= 1 && i <= n - 2) {
+                const float ci = i < 32 ? cpv[i] : cinf;
+                dp = (ei - dp) * ci;
+                d = dp;
+            }
+            v[c] = d;
+        }
+        e_prev = ecar;
+        row_write(S, v);  // the lane's own row
+        __syncthreads();
+        lds_to_global(S, obase, n, nrow, s, n);
+        __syncthreads();
+        if (cb + 1 < nchunk) {
+            tile_to_lds(pf, S);
+            if (cb + 2 < nchunk) tile_fetch(ebase, gstride, nrow, s + 64, n, pf);
+        }
+        __syncthreads();
     }
-    o[1] = y[1] + (u2 - 2.0f * u1);  // u_0 = 0
-    o[0] = y[0] + u1;                // u_{-1} = u_0 = 0
+    // the D rows are read back below by the lanes that wrote them
+    __threadfence();
+
+    // ---- back substitution u_i = d'_i - c'_i u_{i+1} and the outputs ----
+    float u1 = 0.0f, u2 = 0.0f;  // u_{k+1}, u_{k+2} at the top of the tile
+    const int slast = (nchunk - 1) * 32;
+    TileRegs pd, py;
+    tile_fetch(obase, n, nrow, slast, n, pd);
+    tile_fetch(ybase, gstride, nrow, slast, n, py);
+    for (int cb = nchunk - 1; cb >= 0; --cb) {
+        const int s = cb * 32;
+        float d[32], y[32];
+        tile_to_lds(pd, S);
+        __syncthreads();
+        row_read(S, d);
+        __syncthreads();
+        tile_to_lds(py, S);
+        __syncthreads();
+        row_read(S, y);
+        if (cb > 0) {
+            tile_fetch(obase, n, nrow, s - 32, n, pd);
+            tile_fetch(ybase, gstride, nrow, s - 32, n, py);
+        }
+        // c = 31 .. 0: i = s-1+c, k = i+1 = s+c; d[c] = d'_i
+        float o[32];
+#pragma unroll
+        for (int c = 31; c >= 0; --c) {
+            const int i = s - 1 + c;
+            float ui = 0.0f;
+            if (i >= 1 && i <= n - 2) ui = d[c] - (i < 32 ? cpv[i] : cinf) * u1;
+            o[c] = y[c] + (ui - 2.0f * u1 + u2);
+            u2 = u1;
+            u1 = ui;
+        }
+        __syncthreads();  // every lane has read its y row
+        row_write(S, o);
+        __syncthreads();
+        lds_to_global(S, obase, n, nrow, s, n);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill_zero(float *__restrict__ p, int64_t n)
@@ -334,7 +400,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
         if (int rc = check_launch("das_bwd_rf")) return rc;
         if (GK == GK_CUBIC) {
-            hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0,
+            if (a.nt > SPL_MAX_NT) return fail(PYCBF_EINVAL, "spline adjoint: nt too large");
+            hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0,
                                s, a.gy, a.ge, a.gstride, nch, a.nt, grf);
             if (int rc = check_launch("spline_adjoint")) return rc;
         }

@@ -117,6 +117,22 @@ def test_delay_gradient_matches_fp64_central_differences():
             assert abs(g[ai][idx] - fd) <= 1e-3 * max(abs(fd), 1e-2 * peak), (ai, idx, g[ai][idx], fd)
 
 
+@pytest.mark.parametrize("nt", [1, 2, 3, 31, 33, 64, 100, 517])
+def test_cubic_rf_adjoint_trace_lengths(nt):
+    """The transposed spline solve (tiled over 64 channels x 32 samples) for
+    trace lengths around the tile edges, over 70 channels (a ragged second
+    wave): channel-data gradients against FP64 autograd."""
+    print(f"cubic rf adjoint nt={nt}", flush=True)
+    arrs = list(_case(40 + nt, ntx=5, nrx=14, nt=nt, np_=120)[:5])
+    t0, dt = 1e-6, 2.5e-8
+    rng = np.random.default_rng(nt)
+    gout = rng.standard_normal(120).astype(np.float32)
+    _, g = _grads_gpu(arrs, t0, dt, "cubic", "tx_and_rx", 0.0, gout)
+    _, r = _grads_ref(arrs, t0, dt, "cubic", "tx_and_rx", 0.0, gout)
+    peak = max(float(np.abs(r[0]).max()), 1e-30)
+    assert float(np.abs(g[0] - r[0]).max()) <= 1e-4 * peak
+
+
 @pytest.mark.parametrize("kind", ["nearest", "linear", "cubic"])
 def test_rf_adjoint_transpose_identity_full_size(kind):
     """<A rf, G> = <rf, A^T G> for the 31 x 128 x 4096, 256 x 256 workload."""
