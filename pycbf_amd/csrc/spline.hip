@@ -1,12 +1,15 @@
 // Per-channel prefilters: turn each RF channel into a table the DAS gather
 // reads with one load per sample.
 //
-//  * k_spline: natural cubic spline (tie_knots1D_fixed, pycbf/cpu/pycbf.c:114-166).
-//    One wave owns 64 channels; 32-sample tiles are staged through LDS so that
-//    global loads/stores stay coalesced while each lane runs its channel's
-//    sequential recurrence in the reference's exact float32 operation order
-//    (bit-identical second derivatives).  Output is either y2 itself or the
-//    per-interval cubic coefficients used by the DAS gather.
+//  * k_spline_seg (nt <= 4096): natural cubic spline (tie_knots1D_fixed,
+//    pycbf/cpu/pycbf.c:114-166), one wave per channel, each lane sweeping a
+//    segment from a verified carry; bit-identical second derivatives.
+//  * k_spline (longer channels): the same spline, one wave owns 64 channels;
+//    32-sample tiles are staged through LDS so that global loads/stores stay
+//    coalesced while each lane runs its channel's sequential recurrence in the
+//    reference's exact float32 operation order.
+//    Output of both is either y2 itself or the (y, dt^2/6*y2) table used by
+//    the DAS gather.
 //  * k_upsample: the spline evaluated on a dt/usf grid (pycbf/cpu/pycbf.c:234-253).
 //  * k_korder_coefs: Hermite coefficients with k-order derivatives
 //    (pycbf/gpu/__engines__.cu:292-355).
@@ -220,26 +223,323 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
 }
 
-int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
-                        float2 *coef, hipStream_t s)
+// ---- segment-parallel sweeps: one wave per channel -------------------------
+//
+// For nt <= 64*SEG, lane l of the wave owns the SEG consecutive samples
+// i0 = l*SEG .. i0+SEG-1 of one channel, held in registers.  Both sweeps of
+// tie_knots1D_fixed are first-order recurrences that contract by
+// |0.5/p| ~ 0.27 per step (p -> 1 + sqrt(3)/2), so a lane can start each
+// sweep from a guessed carry a little outside its segment:
+//  * forward: u = 0 at i0-32, run over the left neighbour's last 31 samples
+//    to get a guess of u_{i0-1};
+//  * backward: y2 = 0 at i0+SEG+32, run over the right neighbour's first 32
+//    u values to get a guess of y2_{i0+SEG}.
+// After each sweep every guess is compared bit for bit with the value the
+// neighbouring lane computed.  A lane whose guess differs takes its
+// neighbour's value and recomputes its segment, until every carry agrees.
+// Lane 0's forward carry and lane 63's backward carry are the exact boundary
+// values, and a segment computed from an exact carry is exact, so agreement
+// of all carries means every segment equals the sequential sweep bit for bit,
+// whatever the data.  A guess misses only when the data's magnitude falls by
+// ~1e10 within 32 samples (an echo followed by exact zeros); the recomputation
+// costs one segment pass for the few lanes involved.
+//
+// Global memory is touched once per sample: the channel is read in coalesced
+// 64 x 32 tiles (row l = 32 samples of lane l's segment) and transposed
+// through one LDS tile; the output goes back the same way.  Compared with
+// k_spline there is no scratch round trip and no second read of the channel.
+constexpr int SEG_WARM = 32;  // warm-up length of both sweeps (samples)
+
+// Fills u_{i0+c} (c < SEG) from the exact or guessed carry u = u_{i0-1}.
+// y = the lane's samples, yl = y_{i0-1}, yr = y_{i0+SEG}.  u values with
+// index > nt-2 are stored as 0 (the back substitution starts at y2_{nt-1}=0).
+// Returns u_{i0+SEG-1} as seen by the right neighbour.
+template <int SEG>
+__device__ __forceinline__ float seg_forward(const float (&y)[SEG], float yl, float yr, float dx,
+                                             float u, int i0, int nt, float p_inf,
+                                             const float (&plo)[9], bool lane0, float (&v)[SEG])
+{
+    float d_prev = (y[0] - yl) / dx;
+#pragma unroll
+    for (int c = 0; c < SEG; ++c) {
+        const float ynext = (c + 1 < SEG) ? y[c + 1] : yr;
+        const float d = (ynext - y[c]) / dx;
+        const float g = 6.0f * (d - d_prev) / (2.0f * dx);
+        d_prev = d;
+        // p_i = 0.5*y2f[i-1] + 2 differs from p_inf only for i < 9 (lane 0)
+        const float p = (c < 9 && lane0) ? plo[c < 9 ? c : 0] : p_inf;
+        float un = (g - 0.5f * u) / p;
+        if (c == 0) un = lane0 ? 0.0f : un;  // u_0 = 0
+        u = un;
+        v[c] = (i0 + c <= nt - 2) ? u : 0.0f;
+    }
+    return u;
+}
+
+// y2_{i0+c} = y2f[i0+c] * y2_{i0+c+1} + u_{i0+c}, c = SEG-1 .. 0, in place,
+// from the carry y2_{i0+SEG}.  Returns y2_{i0}.
+template <int SEG>
+__device__ __forceinline__ float seg_backward(float (&v)[SEG], float nxt, float f_inf,
+                                              const float (&flo)[8], bool lane0)
+{
+#pragma unroll
+    for (int c = SEG - 1; c >= 0; --c) {
+        const float f = (c < 8 && lane0) ? flo[c < 8 ? c : 0] : f_inf;
+        nxt = f * nxt + v[c];
+        v[c] = nxt;
+    }
+    return nxt;
+}
+
+__device__ __forceinline__ bool bits_differ(float a, float b)
+{
+    return __float_as_uint(a) != __float_as_uint(b);
+}
+
+template <int MODE, int SEG>  // MODE 0: (y, dx^2/6*y2) gather table, 1: y2
+__global__ __launch_bounds__(64) void k_spline_seg(const float *__restrict__ rf, int nt, float dx,
+                                                   void *__restrict__ outp)
+{
+    static_assert(SEG % 32 == 0 && SEG >= SEG_WARM, "segments are whole 32-sample tiles");
+    __shared__ __align__(16) SplRow S[64];
+    const int lane = threadIdx.x;
+    const bool lane0 = lane == 0;
+    const int64_t ch = blockIdx.x;
+    const float *yrow = rf + ch * (int64_t)nt;
+    const int i0 = lane * SEG;
+
+    // forward coefficients y2f[i] = (0.5 - 1) / (0.5*y2f[i-1] + 2), y2f[0] = 0
+    // (pycbf.c:127-133): exact float32 fixed point from i = 7 on
+    float flo[8], plo[9];
+    float y2f_inf;
+    {
+        float w = 0.0f;
+        flo[0] = 0.0f;
+        plo[0] = 0.0f;  // unused: u_0 = 0
+#pragma unroll
+        for (int i = 1; i < 32; ++i) {
+            const float p = 0.5f * w + 2.0f;
+            if (i < 9) plo[i] = p;
+            w = (0.5f - 1.0f) / p;
+            if (i < 8) flo[i] = w;
+        }
+        y2f_inf = w;
+    }
+    const float p_inf = 0.5f * y2f_inf + 2.0f;
+
+    // ---- load: tile t holds samples 32t .. 32t+31 of every segment --------
+    float y[SEG];
+    {
+        const int c = 4 * (lane & 7);
+#pragma unroll
+        for (int t = 0; t < SEG / 32; ++t) {
+            TileRegs r;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = (8 * q + (lane >> 3)) * SEG + 32 * t + c;
+                F4u w{{0.0f, 0.0f, 0.0f, 0.0f}};
+                if (k + 4 <= nt) {
+                    w = *reinterpret_cast<const F4u *>(yrow + k);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (k + e < nt) w.v[e] = yrow[k + e];
+                }
+                r.q[q] = w;
+            }
+            if (t > 0) __syncthreads();  // previous tile read out
+            tile_to_lds(r, S);
+            __syncthreads();
+            float part[32];
+            row_read(S, part);
+#pragma unroll
+            for (int e = 0; e < 32; ++e) y[32 * t + e] = part[e];
+        }
+    }
+    const float yl = __shfl_up(y[SEG - 1], 1);  // y_{i0-1}
+    const float yr = __shfl_down(y[0], 1);      // y_{i0+SEG}
+
+    // ---- forward sweep -----------------------------------------------------
+    // guess of u_{i0-1}: u = 0 at i0-32, then i = i0-31 .. i0-1 over the left
+    // neighbour's last 32 samples (still in the LDS tile)
+    float carry = 0.0f;
+    if (lane > 0) {
+        float w[32];
+#pragma unroll
+        for (int e = 0; e < 32; e += 4) {
+            const float4 q4 = *reinterpret_cast<const float4 *>(&S[lane - 1][e]);
+            w[e] = q4.x;
+            w[e + 1] = q4.y;
+            w[e + 2] = q4.z;
+            w[e + 3] = q4.w;
+        }
+        // only lane 1 of a 32-sample segment reaches i < 9 here
+        const bool low = (SEG == 32) && lane == 1;
+        float d_prev = (w[1] - w[0]) / dx;
+        float u = 0.0f;
+#pragma unroll
+        for (int c = 1; c < 32; ++c) {
+            const float ynext = (c + 1 < 32) ? w[c + 1] : y[0];
+            const float d = (ynext - w[c]) / dx;
+            const float g = 6.0f * (d - d_prev) / (2.0f * dx);
+            d_prev = d;
+            const float p = (c < 9 && low) ? plo[c < 9 ? c : 0] : p_inf;
+            u = (g - 0.5f * u) / p;
+        }
+        carry = u;
+    }
+    float v[SEG];
+    float last = seg_forward<SEG>(y, yl, yr, dx, carry, i0, nt, p_inf, plo, lane0, v);
+    // lanes with i0 > nt-2 hold no u values; their carries need not agree
+    for (;;) {
+        const float want = __shfl_up(last, 1);
+        const bool redo = lane > 0 && i0 <= nt - 2 && bits_differ(carry, want);
+        if (__ballot(redo) == 0) break;
+        if (redo) {
+            carry = want;
+            last = seg_forward<SEG>(y, yl, yr, dx, carry, i0, nt, p_inf, plo, lane0, v);
+        }
+    }
+    const float carry_fwd = carry;
+
+    // ---- back substitution -------------------------------------------------
+    __syncthreads();  // every lane has read its neighbour's y row
+#pragma unroll
+    for (int e = 0; e < 32; e += 4)  // u_{i0 .. i0+31}
+        *reinterpret_cast<float4 *>(&S[lane][e]) = make_float4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+    __syncthreads();
+    float bcarry = 0.0f;  // lane 63: y2 beyond the channel is 0
+    if (lane < 63) {
+        float nxt = 0.0f;
+#pragma unroll
+        for (int e = 28; e >= 0; e -= 4) {
+            const float4 q4 = *reinterpret_cast<const float4 *>(&S[lane + 1][e]);
+            nxt = y2f_inf * nxt + q4.w;
+            nxt = y2f_inf * nxt + q4.z;
+            nxt = y2f_inf * nxt + q4.y;
+            nxt = y2f_inf * nxt + q4.x;
+        }
+        bcarry = nxt;
+    }
+    float first = seg_backward<SEG>(v, bcarry, y2f_inf, flo, lane0);
+    for (;;) {
+        const float want = __shfl_down(first, 1);
+        const bool redo = lane < 63 && i0 < nt && bits_differ(bcarry, want);
+        if (__ballot(redo) == 0) break;
+        if (redo) {
+            // rebuild u from the exact forward carry, then substitute again
+            bcarry = want;
+            seg_forward<SEG>(y, yl, yr, dx, carry_fwd, i0, nt, p_inf, plo, lane0, v);
+            first = seg_backward<SEG>(v, bcarry, y2f_inf, flo, lane0);
+        }
+    }
+
+    // ---- output ------------------------------------------------------------
+    const int c = 4 * (lane & 7);
+    if (MODE == 0) {
+        // 16 samples per pass: lane row = (y_k, h*y2_k) pairs, 32 floats
+        const float h = (1.0f / 6.0f) * (dx * dx);
+        float *tab = static_cast<float *>(outp) + ch * (int64_t)(2 * (nt + 1));
+#pragma unroll
+        for (int hh = 0; hh < SEG / 16; ++hh) {
+            __syncthreads();  // staging tile free
+#pragma unroll
+            for (int j = 0; j < 16; j += 2)
+                *reinterpret_cast<float4 *>(&S[lane][2 * j]) =
+                    make_float4(y[16 * hh + j], h * v[16 * hh + j], y[16 * hh + j + 1],
+                                h * v[16 * hh + j + 1]);
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int r = 8 * q + (lane >> 3);
+                const int f = 2 * (r * SEG + 16 * hh) + c;  // float index in the row
+                const float4 w = *reinterpret_cast<const float4 *>(&S[r][c]);
+                if (f + 4 <= 2 * nt) {
+                    F4u o{{w.x, w.y, w.z, w.w}};
+                    *reinterpret_cast<F4u *>(tab + f) = o;
+                } else {
+                    const float e4[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (f + e < 2 * nt) tab[f + e] = e4[e];
+                }
+            }
+        }
+        if (lane0) {
+            tab[2 * nt] = 0.0f;
+            tab[2 * nt + 1] = 0.0f;
+        }
+    } else {
+        float *y2 = static_cast<float *>(outp) + ch * (int64_t)nt;
+#pragma unroll
+        for (int t = 0; t < SEG / 32; ++t) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 32; e += 4)
+                *reinterpret_cast<float4 *>(&S[lane][e]) =
+                    make_float4(v[32 * t + e], v[32 * t + e + 1], v[32 * t + e + 2],
+                                v[32 * t + e + 3]);
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int r = 8 * q + (lane >> 3);
+                const int k = r * SEG + 32 * t + c;
+                const float4 w = *reinterpret_cast<const float4 *>(&S[r][c]);
+                if (k + 4 <= nt) {
+                    F4u o{{w.x, w.y, w.z, w.w}};
+                    *reinterpret_cast<F4u *>(y2 + k) = o;
+                } else {
+                    const float e4[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (k + e < nt) y2[k + e] = e4[e];
+                }
+            }
+        }
+    }
+}
+
+// Segment length for the one-wave-per-channel kernel, 0 when nt is too long.
+static int spline_seg(int32_t nt)
+{
+    if (nt <= 64 * 32) return 32;
+    if (nt <= 64 * 64) return 64;
+    return 0;
+}
+
+template <int MODE>
+static int launch_spline(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
+                         void *out, hipStream_t s)
 {
     if (nch <= 0 || nt <= 0) return 0;
     if (nt > SPL_MAX_NT) return fail(1, "spline: nt %d exceeds %d samples", nt, SPL_MAX_NT);
-    dim3 grid((unsigned)((nch + 63) / 64));
+    switch (spline_seg(nt)) {
+    case 32:
+        hipLaunchKernelGGL((k_spline_seg<MODE, 32>), dim3((unsigned)nch), dim3(64), 0, s, rf,
+                           (int)nt, dt, out);
+        break;
+    case 64:
+        hipLaunchKernelGGL((k_spline_seg<MODE, 64>), dim3((unsigned)nch), dim3(64), 0, s, rf,
+                           (int)nt, dt, out);
+        break;
+    default:
+        hipLaunchKernelGGL(k_spline<MODE>, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, s, rf,
+                           nch, (int)nt, dt, uscr, out);
+    }
+    return check_launch(MODE == 0 ? "spline coefficients" : "spline y2");
+}
+
+int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr,
+                        float2 *coef, hipStream_t s)
+{
     ProfScope prof("spline_coefs", s);
-    hipLaunchKernelGGL(k_spline<0>, grid, dim3(64), 0, s, rf, nch, (int)nt, dt, uscr,
-                       (void *)coef);
-    return check_launch("spline coefficients");
+    return launch_spline<0>(rf, nch, nt, dt, uscr, (void *)coef, s);
 }
 
 int launch_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr, float *y2,
                      hipStream_t s)
 {
-    if (nch <= 0 || nt <= 0) return 0;
-    if (nt > SPL_MAX_NT) return fail(1, "spline: nt %d exceeds %d samples", nt, SPL_MAX_NT);
-    dim3 grid((unsigned)((nch + 63) / 64));
-    hipLaunchKernelGGL(k_spline<1>, grid, dim3(64), 0, s, rf, nch, (int)nt, dt, uscr, (void *)y2);
-    return check_launch("spline y2");
+    return launch_spline<1>(rf, nch, nt, dt, uscr, (void *)y2, s);
 }
 
 // a^3 rounded once (through double), standing in for powf(a, 3.0f).

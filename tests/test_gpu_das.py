@@ -74,13 +74,18 @@ def test_index_selection_bitwise(c0, kind, usf):
     assert (ref >= 0).mean() > 0.5
 
 
-@pytest.mark.parametrize("data", ["normal", "sparse"])
-@pytest.mark.parametrize("nt", [1, 2, 3, 31, 32, 33, 64, 96, 100, 2048, 4096])
+@pytest.mark.parametrize("data", ["normal", "sparse", "tail", "bursts"])
+@pytest.mark.parametrize("nt", [1, 2, 3, 31, 32, 33, 64, 96, 100, 2047, 2048, 2049, 4095, 4096,
+                                4097, 6000])
 def test_spline_prefilter_bitwise(nt, data):
     """Second derivatives (pycbf_spline_y2) and the gather table
     (pycbf_spline_coefs) are bit-identical to tie_knots1D_fixed's, for dense
-    data and for sparse spikes between exact zeros (whose recurrence tails
-    decay through the subnormal range)."""
+    data, for sparse spikes between exact zeros (whose recurrence tails decay
+    through the subnormal range), for echoes followed by a long run of zeros,
+    and for short bursts between zero runs of every length.  Up to 4096
+    samples one wave sweeps each channel in segments whose guessed carries
+    are checked and recomputed (the zero runs force recomputation); longer
+    channels take the one-lane-per-channel kernel."""
     import torch
     from pycbf_amd.gpu import _engine as E
     print(f"spline nt={nt} data={data}", flush=True)
@@ -89,6 +94,19 @@ def test_spline_prefilter_bitwise(nt, data):
     rf = rng.standard_normal((nch, nt)).astype(np.float32)
     if data == "sparse":
         rf *= (rng.random((nch, nt)) < 0.01).astype(np.float32)
+    elif data == "tail":
+        # channel c keeps its first 1 + (c * 37) % nt samples
+        keep = 1 + (np.arange(nch) * 37) % max(nt, 1)
+        rf[np.arange(nt)[None, :] >= keep[:, None]] = 0.0
+    elif data == "bursts":
+        on = np.zeros((nch, nt), dtype=bool)
+        for c in range(nch):
+            k = 0
+            while k < nt:
+                n_on = int(rng.integers(1, 8))
+                on[c, k:k + n_on] = True
+                k += n_on + int(rng.integers(1, 200))
+        rf *= on.astype(np.float32)
     dev = torch.device("cuda")
     x = torch.as_tensor(rf, device=dev)
     wsb = E.lib.pycbf_spline_workspace_bytes(nch, nt)
