@@ -35,6 +35,8 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "common.h"
 #include "prof.h"
 
@@ -46,11 +48,51 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
+// value of lane ^ 1 / lane ^ 2 (DPP quad permutations)
+__device__ __forceinline__ float dpp_xor1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_xor2(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+
+// sum over the wave, in every lane: DPP within each row of 16 lanes
+// (quad swaps, half-row and row mirrors), then across the four rows
 __device__ __forceinline__ float wave_sum(float v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    v += dpp_xor1(v);
+    v += dpp_xor2(v);
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, true));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, true));
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
     return v;
+}
+
+// Per-lane constants of the band extraction of a 32x32 MFMA accumulator
+// block: lane (j + 32h) holds G[i][j] in register r, i = 8(r/4) + 4h + r%4.
+// Entry r of a lane is taken from G0 when j >= i (tsel >= 8(r/4) + r%4 with
+// tsel = j - 4h) and is fetched by lane 32h + ((j + i) & 31), whose byte
+// address for ds_bpermute is baddr[r].
+struct BandLanes {
+    int tsel;
+    int baddr[16];
+};
+
+__device__ __forceinline__ BandLanes band_lanes()
+{
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 31, h = lane >> 5;
+    BandLanes b;
+    b.tsel = j - 4 * h;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+        b.baddr[r] = (32 * h + ((j + i) & 31)) << 2;
+    }
+    return b;
 }
 
 __device__ __forceinline__ unsigned lanes_below(unsigned long long mask)
@@ -96,6 +138,186 @@ struct __attribute__((aligned(4))) F4s {
     float v[4];
 };
 
+// Lag sums and output of one pixel from its normalised, compacted channels:
+// n usable channels in zbuf (FP32 rows of KP+1 floats, or FP16 / BF16 rows of
+// KP halves), their validity bytes in vflag.  Wave-uniform arguments; every
+// lane of the wave calls it.
+// One 32x32x8 product of normalised FP16 / BF16 channels.
+template <int PREC, typename T>
+__device__ __forceinline__ f32x16_t mfma_k8(const T &a, const T &b, const f32x16_t &c)
+{
+    if constexpr (PREC == 1)
+        return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// NBMAX: blocks of 32 channels a pixel can have (0: operands read per product)
+template <int PREC, int KP, int NBMAX = 0>
+__device__ __forceinline__ void slsc_pixel(const SlscArgs &a, unsigned char *zbuf,
+                                           const unsigned char *vflag, float *red, int n,
+                                           const BandLanes &bl, float *dst)
+{
+    const int lane = threadIdx.x & 63;
+    if (n < a.need) {
+        if (lane == 0) *dst = 0.0f;
+        return;
+    }
+    // valid-channel bits in selected order
+    unsigned long long V0 = 0ull, V1 = 0ull, V2 = 0ull, V3 = 0ull;
+    V0 = __ballot(lane < n && vflag[lane]);
+    if (n > 64) V1 = __ballot(64 + lane < n && vflag[64 + lane]);
+    if (n > 128) V2 = __ballot(128 + lane < n && vflag[128 + lane]);
+    if (n > 192) V3 = __ballot(192 + lane < n && vflag[192 + lane]);
+
+    float S = 0.0f;  // lane li: S of lag lags[li]
+    if (PREC == 0) {
+        // lags in chunks of 8: per-lane partial sums in registers, one LDS
+        // transpose-and-add per chunk gives the 8 wave totals
+        const float *zb = reinterpret_cast<const float *>(zbuf);
+        for (int lc = 0; lc < a.nlag; lc += 8) {
+            int mm[8];
+            float acc[8];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                mm[l] = lc + l < a.nlag ? a.lags[lc + l] : 4096;
+                acc[l] = 0.0f;
+            }
+            for (int pc = 0; pc < n; pc += 64) {
+                const int p = pc + lane;
+                const bool act = p < n;
+                float zr[KP];
+#pragma unroll
+                for (int w = 0; w < KP; ++w) zr[w] = act ? zb[p * (KP + 1) + w] : 0.0f;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) {
+                    if (act && p + mm[l] < n) {
+                        const float *zq = zb + (p + mm[l]) * (KP + 1);
+                        float s = 0.0f;
+#pragma unroll
+                        for (int w = 0; w < KP; ++w) s = __builtin_fmaf(zr[w], zq[w], s);
+                        acc[l] += s;
+                    }
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < 8; ++l) red[l * 65 + lane] = acc[l];
+            const int l8 = lane >> 3, part = lane & 7;
+            float s = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s += red[l8 * 65 + part * 8 + i];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            s += __shfl_xor(s, 4);
+            const float got = __shfl(s, ((lane - lc) & 7) * 8);
+            if (lane >= lc && lane < lc + 8 && lane < a.nlag) S = got;
+        }
+    } else {
+        // zero rows n .. 32*nblk-1 of the last block
+        const int nblk = (n + 31) >> 5;
+        unsigned short *zr16 = reinterpret_cast<unsigned short *>(zbuf);
+        for (int t = n * KP + lane; t < nblk * 32 * KP; t += 64) zr16[t] = 0;
+        const int j = lane & 31, h = lane >> 5;
+        const int mylag = lane < a.nlag ? a.lags[lane] : 0;
+        // lags are at most 63: one or two lag groups
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (q >= a.ngroups) break;
+            // The band extraction below is linear and the same for every
+            // block pair, so the Gram blocks (b, b+q) and (b, b+q+1) of all
+            // b are summed in the MFMA accumulators and extracted once.
+            // the first product of each accumulator starts from zero (an
+            // inline constant operand, no accumulator initialisation)
+            const f32x16_t zero = {};
+            f32x16_t g0 = zero, g1 = zero;
+            if constexpr (KP == 8 && NBMAX > 0) {
+                // every block's operand (this lane's row and half-window) is
+                // read before the first product, so the products issue back
+                // to back instead of waiting on LDS between them
+                typedef typename std::conditional<PREC == 1, half4_t, bf16x4_t>::type op_t;
+                op_t ops[NBMAX];
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b)
+                    if (b < nblk) ops[b] = *reinterpret_cast<const op_t *>(zr16 + (32 * b + j) * KP + 4 * h);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    if (b + q < nblk) {
+                        g0 = mfma_k8<PREC>(ops[b], ops[b + q < NBMAX ? b + q : 0], b == 0 ? zero : g0);
+                        if (b + q + 1 < nblk)
+                            g1 = mfma_k8<PREC>(ops[b], ops[b + q + 1 < NBMAX ? b + q + 1 : 0],
+                                               b == 0 ? zero : g1);
+                    }
+                }
+            } else {
+            for (int b = 0; b + q < nblk; ++b) {
+                const bool two = b + q + 1 < nblk;
+#pragma unroll
+                for (int kc = 0; kc < KP / 8; ++kc) {
+                    const int ko = kc * 8 + 4 * h;
+                    const unsigned short *pa = zr16 + (32 * b + j) * KP + ko;
+                    const unsigned short *pb0 = zr16 + (32 * (b + q) + j) * KP + ko;
+                    const unsigned short *pb1 = zr16 + (32 * (b + q + 1) + j) * KP + ko;
+                    const bool first = b == 0 && kc == 0;
+                    if (PREC == 1) {
+                        const half4_t va = *reinterpret_cast<const half4_t *>(pa);
+                        const half4_t vb0 = *reinterpret_cast<const half4_t *>(pb0);
+                        g0 = first ? __builtin_amdgcn_mfma_f32_32x32x8f16(va, vb0, zero, 0, 0, 0)
+                                   : __builtin_amdgcn_mfma_f32_32x32x8f16(va, vb0, g0, 0, 0, 0);
+                        if (two) {
+                            const half4_t vb1 = *reinterpret_cast<const half4_t *>(pb1);
+                            g1 = first
+                                     ? __builtin_amdgcn_mfma_f32_32x32x8f16(va, vb1, zero, 0, 0, 0)
+                                     : __builtin_amdgcn_mfma_f32_32x32x8f16(va, vb1, g1, 0, 0, 0);
+                        }
+                    } else {
+                        const bf16x4_t va = *reinterpret_cast<const bf16x4_t *>(pa);
+                        const bf16x4_t vb0 = *reinterpret_cast<const bf16x4_t *>(pb0);
+                        g0 = first ? __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb0, zero, 0,
+                                                                               0, 0)
+                                   : __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb0, g0, 0, 0,
+                                                                               0);
+                        if (two) {
+                            const bf16x4_t vb1 = *reinterpret_cast<const bf16x4_t *>(pb1);
+                            g1 = first ? __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb1, zero,
+                                                                                   0, 0, 0)
+                                       : __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb1, g1, 0,
+                                                                                   0, 0);
+                        }
+                    }
+                }
+            }
+            }
+            float acc = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float v = bl.tsel >= 8 * (r >> 2) + (r & 3) ? g0[r] : g1[r];
+                acc += __int_as_float(__builtin_amdgcn_ds_bpermute(bl.baddr[r], __float_as_int(v)));
+            }
+            acc += __shfl_xor(acc, 32);
+            const float got = __shfl(acc, mylag & 31);
+            if (lane < a.nlag && (mylag >> 5) == q) S = got;
+        }
+    }
+
+    const bool allvalid = __popcll(V0) + __popcll(V1) + __popcll(V2) + __popcll(V3) == n;
+    float contrib = 0.0f;
+    if (lane < a.nlag) {
+        const int cnt = allvalid ? max(n - a.lags[lane], 0) : pair_count(V0, V1, V2, V3, a.lags[lane]);
+        const float rho = cnt > 0 ? S / (float)cnt : __builtin_nanf("");
+        contrib = rho * a.inv_nlag;
+    }
+    contrib = wave_sum(contrib);
+    if (lane == 0) *dst = contrib;
+}
+
+// normalised sample z in the zbuf element type of PREC
+template <int PREC>
+__device__ __forceinline__ unsigned zcode(float z)
+{
+    return PREC == 1 ? (unsigned)__builtin_bit_cast(unsigned short, (_Float16)z)
+                     : (unsigned)to_bf16(z);
+}
+
 // One wave per run of pz consecutive output pixels of a column; no
 // workgroup-level staging or barriers.  Lane l owns channels l, l+64, ...
 // (NG groups of 64) and loads each channel's window straight from the image:
@@ -117,6 +339,7 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
     const int64_t rstride = a.nx * a.nz;
     const float *col = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;  // + r * rstride + z
     const int izo0 = (blockIdx.x * nwave + wave) * a.pz;
+    const BandLanes bl = band_lanes();
 
     for (int k = 0; k < a.pz; ++k) {
         const int izo = izo0 + k;
@@ -191,8 +414,7 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
 #pragma unroll
                     for (int w = 0; w < KP; ++w) {
                         const float z = w < keff ? (xv[g][w] - mu) * rs : 0.0f;
-                        hv[w] = PREC == 1 ? (unsigned)__builtin_bit_cast(unsigned short, (_Float16)z)
-                                          : (unsigned)to_bf16(z);
+                        hv[w] = zcode<PREC>(z);
                     }
 #pragma unroll
                     for (int w = 0; w < KP; w += 4)
@@ -204,119 +426,211 @@ __global__ __launch_bounds__(256) void k_slsc(SlscArgs a)
         }
 
         float *dst = a.out + ((int64_t)f * a.nxo + ixo) * a.nzo + izo;
-        if (n < a.need) {
-            if (lane == 0) *dst = 0.0f;
-            continue;
-        }
-        // valid-channel bits in selected order
-        unsigned long long V0 = 0ull, V1 = 0ull, V2 = 0ull, V3 = 0ull;
-        V0 = __ballot(lane < n && vflag[lane]);
-        if (n > 64) V1 = __ballot(64 + lane < n && vflag[64 + lane]);
-        if (n > 128) V2 = __ballot(128 + lane < n && vflag[128 + lane]);
-        if (n > 192) V3 = __ballot(192 + lane < n && vflag[192 + lane]);
+        slsc_pixel<PREC, KP>(a, zbuf, vflag, red, n, bl, dst);
+    }
+}
 
-        float S = 0.0f;  // lane li: S of lag lags[li]
-        if (PREC == 0) {
-            // lags in chunks of 8: per-lane partial sums in registers, one LDS
-            // transpose-and-add per chunk gives the 8 wave totals
-            const float *zb = reinterpret_cast<const float *>(zbuf);
-            for (int lc = 0; lc < a.nlag; lc += 8) {
-                int mm[8];
-                float acc[8];
+// sum over the LPP lanes (aligned groups) that hold one window
+template <int LPP>
+__device__ __forceinline__ float lpp_sum(float v)
+{
+    if (LPP >= 2) v += dpp_xor1(v);
+    if (LPP >= 4) v += dpp_xor2(v);
+    if (LPP >= 8) v += __shfl_xor(v, 4);
+    return v;
+}
+
+// Fast path of the MFMA precisions for windows that tile the axis
+// (dsfz == kwin == KP): a wave takes PZ = 32/KP consecutive output pixels of a
+// column, whose windows are 32 consecutive samples of every channel.  Those
+// are read as channel rows: lane l holds samples 4*(l%8) .. +3 of channel
+// 8q + l/8, so one load instruction covers eight channels x 128 contiguous
+// bytes (lane-per-channel loads of 16 bytes make each lane its own L2
+// request), and a window of pixel p is held by the KP/4 lanes l%8 = p*KP/4 ..
+// of each row.  For K = 8 the two lanes of a window swap halves (DPP) so that
+// each lane normalises one whole window of two rows; for K = 16 / 32 the
+// lanes of a window reduce its mean and variance with shuffles.  The
+// normalised samples go to the pixel's compacted buffer, and each pixel's lag
+// sums then run as in k_slsc with every block operand read before the first
+// product.  The wave's selection words are read with scalar loads.
+template <int PREC, int KP, int NG>
+__global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
+{
+    static_assert(PREC != 0, "the channel-row path feeds the MFMA kernels");
+    constexpr int PZ = 32 / KP, LPP = KP / 4, NQ = NG * 8;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nrx = (int)a.nrx;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
+    unsigned char *wbase = smem + (size_t)wave * a.wbytes;  // PZ x (zbuf, flags)
+    float *red = nullptr;  // FP32 reduction rows: not used by the MFMA kernels
+
+    const int f = blockIdx.z, ixo = blockIdx.y;
+    const int ix = ixo * a.dsfx;
+    const int64_t rstride = a.nx * a.nz;
+    const float *col = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;  // + r * rstride + z
+    const int izo0 = (blockIdx.x * nwave + wave) * PZ;
+
+    const int c8 = lane >> 3, k8 = lane & 7;
+    const int pp = k8 / LPP;                  // this lane's pixel
+    const int izo = izo0 + pp;
+    const bool pix = izo < a.nzo;
+    const int zl = izo0 * a.dsfz + a.kofs + 4 * k8;  // first sample of this lane
+    // the window of pixel pp clipped to the image: keff samples
+    const int wz0 = izo * a.dsfz + a.kofs;
+    const int keff = min(wz0 + KP, (int)a.nz) - max(wz0, 0);
+    const float rk = 1.0f / (float)keff;
+    const float vmin = 1e-20f * (float)keff;
+    bool inr[4];
 #pragma unroll
-                for (int l = 0; l < 8; ++l) {
-                    mm[l] = lc + l < a.nlag ? a.lags[lc + l] : 4096;
-                    acc[l] = 0.0f;
-                }
-                for (int pc = 0; pc < n; pc += 64) {
-                    const int p = pc + lane;
-                    const bool act = p < n;
-                    float zr[KP];
+    for (int e = 0; e < 4; ++e) inr[e] = zl + e >= 0 && zl + e < a.nz;
+    const bool inner = zl >= 0 && zl + 4 <= a.nz;
+
+    // usable-element bits of this lane's pixel: the wave's PZ pixels are
+    // consecutive, so their words are read with uniform (scalar) loads
+    unsigned long long m[NG];
+    {
+        const unsigned long long *sw = a.sel ? a.sel + ((int64_t)ixo * a.nzo + izo0) * 4 : nullptr;
 #pragma unroll
-                    for (int w = 0; w < KP; ++w) zr[w] = act ? zb[p * (KP + 1) + w] : 0.0f;
+        for (int g = 0; g < NG; ++g) {
+            const int rem = nrx - 64 * g;
+            const unsigned long long all =
+                rem >= 64 ? ~0ull : (rem <= 0 ? 0ull : ((1ull << rem) - 1ull));
+            unsigned long long w = 0ull;
 #pragma unroll
-                    for (int l = 0; l < 8; ++l) {
-                        if (act && p + mm[l] < n) {
-                            const float *zq = zb + (p + mm[l]) * (KP + 1);
-                            float s = 0.0f;
-#pragma unroll
-                            for (int w = 0; w < KP; ++w) s = __builtin_fmaf(zr[w], zq[w], s);
-                            acc[l] += s;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int l = 0; l < 8; ++l) red[l * 65 + lane] = acc[l];
-                const int l8 = lane >> 3, part = lane & 7;
-                float s = 0.0f;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) s += red[l8 * 65 + part * 8 + i];
-                s += __shfl_xor(s, 1);
-                s += __shfl_xor(s, 2);
-                s += __shfl_xor(s, 4);
-                const float got = __shfl(s, ((lane - lc) & 7) * 8);
-                if (lane >= lc && lane < lc + 8 && lane < a.nlag) S = got;
+            for (int p = 0; p < PZ; ++p) {
+                const unsigned long long wp =
+                    izo0 + p < a.nzo ? (sw ? sw[p * 4 + g] & all : all) : 0ull;
+                w = pp == p ? wp : w;
             }
-        } else {
-            // zero rows n .. 32*nblk-1 of the last block
-            const int nblk = (n + 31) >> 5;
-            unsigned short *zr16 = reinterpret_cast<unsigned short *>(zbuf);
-            for (int t = n * KP + lane; t < nblk * 32 * KP; t += 64) zr16[t] = 0;
-            const int j = lane & 31, h = lane >> 5;
-            const int mylag = lane < a.nlag ? a.lags[lane] : 0;
-            for (int q = 0; q < a.ngroups; ++q) {
-                // The band extraction below is linear and the same for every
-                // block pair, so the Gram blocks (b, b+q) and (b, b+q+1) of all
-                // b are summed in the MFMA accumulators and extracted once.
-                f32x16_t g0 = {}, g1 = {};
-                for (int b = 0; b + q < nblk; ++b) {
-                    const bool two = b + q + 1 < nblk;
+            m[g] = w;
+        }
+    }
+
+    // channel rows: all loads in flight before the first use
+    float x[NQ][4];
 #pragma unroll
-                    for (int kc = 0; kc < KP / 8; ++kc) {
-                        const int ko = kc * 8 + 4 * h;
-                        const unsigned short *pa = zr16 + (32 * b + j) * KP + ko;
-                        const unsigned short *pb0 = zr16 + (32 * (b + q) + j) * KP + ko;
-                        const unsigned short *pb1 = zr16 + (32 * (b + q + 1) + j) * KP + ko;
-                        if (PREC == 1) {
-                            const half4_t va = *reinterpret_cast<const half4_t *>(pa);
-                            g0 = __builtin_amdgcn_mfma_f32_32x32x8f16(
-                                va, *reinterpret_cast<const half4_t *>(pb0), g0, 0, 0, 0);
-                            if (two)
-                                g1 = __builtin_amdgcn_mfma_f32_32x32x8f16(
-                                    va, *reinterpret_cast<const half4_t *>(pb1), g1, 0, 0, 0);
-                        } else {
-                            const bf16x4_t va = *reinterpret_cast<const bf16x4_t *>(pa);
-                            g0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-                                va, *reinterpret_cast<const bf16x4_t *>(pb0), g0, 0, 0, 0);
-                            if (two)
-                                g1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-                                    va, *reinterpret_cast<const bf16x4_t *>(pb1), g1, 0, 0, 0);
-                        }
-                    }
-                }
-                float acc = 0.0f;
+    for (int q = 0; q < NQ; ++q) {
+        const int r = 8 * q + c8;
+        const bool sel = (m[q / 8] >> (r & 63)) & 1ull;
+        const float *xr = col + (int64_t)(sel ? r : 0) * rstride + zl;
+        F4s v{{0.0f, 0.0f, 0.0f, 0.0f}};
+        if (sel) {
+            if (inner) {
+                v = *reinterpret_cast<const F4s *>(xr);
+            } else {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-                    const float v = j >= i ? g0[r] : g1[r];
-                    const int srcl = 32 * h + ((j + i) & 31);
-                    acc += __int_as_float(
-                        __builtin_amdgcn_ds_bpermute(srcl << 2, __float_as_int(v)));
-                }
-                acc += __shfl_xor(acc, 32);
-                const float got = __shfl(acc, mylag & 31);
-                if (lane < a.nlag && (mylag >> 5) == q) S = got;
+                for (int e = 0; e < 4; ++e)
+                    if (inr[e]) v.v[e] = xr[e];
             }
         }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[q][e] = v.v[e];
+    }
 
-        float contrib = 0.0f;
-        if (lane < a.nlag) {
-            const int cnt = pair_count(V0, V1, V2, V3, a.lags[lane]);
-            const float rho = cnt > 0 ? S / (float)cnt : __builtin_nanf("");
-            contrib = rho * a.inv_nlag;
+    unsigned char *zb = wbase + pp * (a.zbytes + 256);
+    unsigned char *vf = zb + a.zbytes;
+    if constexpr (LPP == 2) {
+        // Channel rows q0 = 2t and q1 = 2t+1: the two lanes holding the
+        // halves of a window swap one half (DPP), so that lane (c8, k8)
+        // holds the whole 8-sample window of channel 8*(2t + k8%2) + c8 for
+        // its pixel k8/2 and normalises it in registers.
+        const int hh = k8 & 1;
+        bool inr8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) inr8[e] = wz0 + e >= 0 && wz0 + e < a.nz;
+#pragma unroll
+        for (int t = 0; t < NQ / 2; ++t) {
+            float xw[8];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float rcv = dpp_xor1(hh ? x[2 * t][e] : x[2 * t + 1][e]);
+                xw[e] = hh ? rcv : x[2 * t][e];
+                xw[4 + e] = hh ? x[2 * t + 1][e] : rcv;
+            }
+            const int r = 8 * (2 * t + hh) + c8;
+            const bool sel = (m[t / 4] >> (r & 63)) & 1ull;
+            float s8 = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s8 += xw[e];
+            const float mu = s8 * rk;
+            float d[8];
+            float v = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                d[e] = inr8[e] ? xw[e] - mu : 0.0f;
+                v = __builtin_fmaf(d[e], d[e], v);
+            }
+            const bool valid = v >= vmin;
+            const float rs = valid ? __frsqrt_rn(v) : 0.0f;
+            if (sel) {
+                int pos = __popcll(m[t / 4] & ((1ull << (r & 63)) - 1ull));
+#pragma unroll
+                for (int g = 0; g < t / 4; ++g) pos += __popcll(m[g]);
+                vf[pos] = valid ? 1 : 0;
+                unsigned hv[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) hv[e] = zcode<PREC>(d[e] * rs);
+                *reinterpret_cast<uint4 *>(zb + (size_t)pos * 16) =
+                    make_uint4(hv[0] | (hv[1] << 16), hv[2] | (hv[3] << 16),
+                               hv[4] | (hv[5] << 16), hv[6] | (hv[7] << 16));
+            }
         }
-        contrib = wave_sum(contrib);
-        if (lane == 0) *dst = contrib;
+    } else {
+        // normalise each (channel, pixel) window over its LPP lanes and write it
+        // to the pixel's compacted buffer
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = 8 * q + c8;
+            const bool sel = (m[q / 8] >> (r & 63)) & 1ull;
+            float s = (x[q][0] + x[q][1]) + (x[q][2] + x[q][3]);
+            s = lpp_sum<LPP>(s);
+            const float mu = s * rk;
+            float d[4];
+            float v = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                d[e] = inr[e] ? x[q][e] - mu : 0.0f;
+                v = __builtin_fmaf(d[e], d[e], v);
+            }
+            v = lpp_sum<LPP>(v);
+            const bool valid = v >= vmin;
+            const float rs = valid ? __frsqrt_rn(v) : 0.0f;
+            if (sel) {
+                // position among the pixel's usable channels
+                int pos = __popcll(m[q / 8] & ((1ull << (r & 63)) - 1ull));
+#pragma unroll
+                for (int g = 0; g < q / 8; ++g) pos += __popcll(m[g]);
+                if (k8 % LPP == 0) vf[pos] = valid ? 1 : 0;
+                const int w0 = 4 * (k8 % LPP);
+                unsigned *zr = reinterpret_cast<unsigned *>(zb) + (pos * KP + w0) / 2;
+                *reinterpret_cast<uint2 *>(zr) =
+                    make_uint2(zcode<PREC>(d[0] * rs) | (zcode<PREC>(d[1] * rs) << 16),
+                               zcode<PREC>(d[2] * rs) | (zcode<PREC>(d[3] * rs) << 16));
+            }
+        }
+
+    }
+    __syncthreads();  // compacted buffers complete (all waves reach this once)
+    const BandLanes bl = band_lanes();
+
+#pragma unroll
+    for (int p = 0; p < PZ; ++p) {
+        const int izp = izo0 + p;
+        if (izp >= a.nzo) break;
+        // usable channels of pixel p: the bits held by its lanes
+        // (readlane yields int: take both halves as unsigned before widening)
+        int n = 0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)m[g], p * LPP);
+            const unsigned hi =
+                (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(m[g] >> 32), p * LPP);
+            n += __popc(lo) + __popc(hi);
+        }
+        unsigned char *zbp = wbase + p * (a.zbytes + 256);
+        float *dst = a.out + ((int64_t)f * a.nxo + ixo) * a.nzo + izp;
+        slsc_pixel<PREC, KP, 2 * NG>(a, zbp, zbp + a.zbytes, red, n, bl, dst);
     }
 }
 
@@ -359,48 +673,70 @@ int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t 
 }
 
 // Per-wave LDS (normalised channels, flags, FP32 reduction rows), waves per
-// workgroup and pixels per wave of a launch.
-static void slsc_plan(SlscArgs &a, int prec, int kp, int &nwave, size_t &lds)
+// workgroup and pixels per wave of a launch.  rows: the channel-row fast path
+// (PZ = 32/KP pixels per wave, one buffer per pixel).
+static void slsc_plan(SlscArgs &a, int prec, int kp, bool rows, int &nwave, size_t &lds)
 {
     const int nrx = (int)a.nrx;
     const int npad = (nrx + 31) / 32 * 32;
     a.zbytes = (int)align_up(prec == 0 ? (size_t)npad * (kp + 1) * 4 : (size_t)npad * kp * 2, 16);
-    a.wbytes = (int)align_up((size_t)a.zbytes + 256 + (prec == 0 ? 8 * 65 * 4 : 0), 16);
-    // 2 pixels per wave: more waves in flight beat longer runs per wave
-    // (measured on 128 channels, K = 8, dsfz = 8: 2 px 76 us, 4 px 92 us, 8 px
-    // 154 us for the FP16 kernel)
-    int pz = 2;
-    if (const char *e = getenv("PYCBF_SLSC_PZ")) pz = std::max(1, std::min(64, atoi(e)));
-    a.pz = pz;
-    nwave = 4;
+    const size_t redb = prec == 0 ? 8 * 65 * 4 : 0;
+    if (rows) {
+        // MFMA precisions only
+        a.pz = 32 / kp;
+        a.wbytes = (int)align_up((size_t)a.pz * (a.zbytes + 256), 16);
+        // up to 40 KB per workgroup keeps four workgroups per CU
+        nwave = std::max(1, std::min(4, 40960 / a.wbytes));
+    } else {
+        a.wbytes = (int)align_up((size_t)a.zbytes + 256 + redb, 16);
+        // one pixel per wave: more waves in flight beat longer runs per wave
+        // (measured on 128 channels, K = 8, dsfz = 8: 1 px 69 us, 2 px 76 us,
+        // 4 px 92 us for the FP16 kernel)
+        int pz = 1;
+        if (const char *e = getenv("PYCBF_SLSC_PZ")) pz = std::max(1, std::min(64, atoi(e)));
+        a.pz = pz;
+        nwave = 4;
+    }
     lds = (size_t)nwave * a.wbytes;
 }
 
+static const char *slsc_prof_name(int prec)
+{
+    return prec == 0 ? "slsc_fp32" : (prec == 1 ? "slsc_mfma_fp16" : "slsc_mfma_bf16");
+}
+
 template <int PREC, int KP, int NG>
-static int launch_slsc_t(const SlscArgs &a, int nwave, size_t lds, int64_t nfr, hipStream_t s)
+static int launch_slsc_t(const SlscArgs &a, bool rows, int nwave, size_t lds, int64_t nfr,
+                         hipStream_t s)
 {
     const int per_block = a.pz * nwave;
     const dim3 grid((unsigned)((a.nzo + per_block - 1) / per_block), (unsigned)a.nxo, (unsigned)nfr);
-    ProfScope ps(PREC == 0 ? "slsc_fp32" : (PREC == 1 ? "slsc_mfma_fp16" : "slsc_mfma_bf16"), s);
+    ProfScope ps(slsc_prof_name(PREC), s);
+    if constexpr (PREC != 0) {
+        if (rows) {
+            hipLaunchKernelGGL((k_slsc_rows<PREC, KP, NG>), grid, dim3(64 * nwave), lds, s, a);
+            return check_launch("slsc");
+        }
+    }
     hipLaunchKernelGGL((k_slsc<PREC, KP, NG>), grid, dim3(64 * nwave), lds, s, a);
     return check_launch("slsc");
 }
 
 template <int PREC>
-static int launch_slsc_p(const SlscArgs &a, int kp, int nwave, size_t lds, int64_t nfr,
-                         hipStream_t s)
+static int launch_slsc_p(const SlscArgs &a, int kp, bool rows, int nwave, size_t lds,
+                         int64_t nfr, hipStream_t s)
 {
     if (a.nrx <= 128) {
         switch (kp) {
-        case 8: return launch_slsc_t<PREC, 8, 2>(a, nwave, lds, nfr, s);
-        case 16: return launch_slsc_t<PREC, 16, 2>(a, nwave, lds, nfr, s);
-        default: return launch_slsc_t<PREC, 32, 2>(a, nwave, lds, nfr, s);
+        case 8: return launch_slsc_t<PREC, 8, 2>(a, rows, nwave, lds, nfr, s);
+        case 16: return launch_slsc_t<PREC, 16, 2>(a, rows, nwave, lds, nfr, s);
+        default: return launch_slsc_t<PREC, 32, 2>(a, rows, nwave, lds, nfr, s);
         }
     }
     switch (kp) {
-    case 8: return launch_slsc_t<PREC, 8, 4>(a, nwave, lds, nfr, s);
-    case 16: return launch_slsc_t<PREC, 16, 4>(a, nwave, lds, nfr, s);
-    default: return launch_slsc_t<PREC, 32, 4>(a, nwave, lds, nfr, s);
+    case 8: return launch_slsc_t<PREC, 8, 4>(a, rows, nwave, lds, nfr, s);
+    case 16: return launch_slsc_t<PREC, 16, 4>(a, rows, nwave, lds, nfr, s);
+    default: return launch_slsc_t<PREC, 32, 4>(a, rows, nwave, lds, nfr, s);
     }
 }
 
@@ -431,13 +767,17 @@ int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t n
     a.ngroups = lagmax / 32 + 1;
     a.inv_nlag = 1.0f / (float)nlag;
     const int kp = kwin <= 8 ? 8 : (kwin <= 16 ? 16 : 32);
+    // windows that tile the axis take the channel-row path (PYCBF_SLSC_ROWS=0
+    // selects the lane-per-channel kernel)
+    bool rows = precision != PYCBF_SLSC_FP32 && dsfz == kwin && kwin == kp;
+    if (const char *e = getenv("PYCBF_SLSC_ROWS")) rows = rows && atoi(e) != 0;
     int nwave = 0;
     size_t lds = 0;
-    slsc_plan(a, precision, kp, nwave, lds);
+    slsc_plan(a, precision, kp, rows, nwave, lds);
     switch (precision) {
-    case PYCBF_SLSC_FP32: return launch_slsc_p<0>(a, kp, nwave, lds, nfr, s);
-    case PYCBF_SLSC_MFMA_FP16: return launch_slsc_p<1>(a, kp, nwave, lds, nfr, s);
-    default: return launch_slsc_p<2>(a, kp, nwave, lds, nfr, s);
+    case PYCBF_SLSC_FP32: return launch_slsc_p<0>(a, kp, rows, nwave, lds, nfr, s);
+    case PYCBF_SLSC_MFMA_FP16: return launch_slsc_p<1>(a, kp, rows, nwave, lds, nfr, s);
+    default: return launch_slsc_p<2>(a, kp, rows, nwave, lds, nfr, s);
     }
 }
 

@@ -45,6 +45,13 @@ CASES = {
     "40rx_k3_few_elements": (40, 3, 20, 3, -1, 1, 1, [1, 2, 5], True),
     "96rx_k16_lags1-20": (96, 3, 48, 16, -8, 1, 4, list(range(1, 21)), False),
     "160rx_k32_apod": (160, 2, 70, 32, -16, 1, 8, [1, 2, 4, 8, 16], True),
+    # windows that tile the axis (dsfz == kwin): the channel-row kernel, with
+    # windows clipped at both ends, partial runs of pixels, unaligned offsets
+    "128rx_k8_tiles_lags1-25": (128, 3, 64, 8, -4, 1, 8, list(range(1, 26)), False),
+    "200rx_k8_tiles_apod": (200, 4, 50, 8, -3, 2, 8, list(range(1, 31)), True),
+    "100rx_k16_tiles_apod": (100, 2, 70, 16, -8, 1, 16, list(range(1, 13)), True),
+    "64rx_k32_tiles": (64, 2, 100, 32, -16, 1, 32, [1, 2, 4, 8], False),
+    "256rx_k8_tiles_sparse_lags": (256, 2, 40, 8, 0, 1, 8, [1, 3, 7, 31, 32, 33, 63], False),
 }
 
 
