@@ -44,6 +44,10 @@ int launch_spline_coefs(const float *rf, int64_t nch, int32_t nt, float dt, floa
                         float2 *coef, hipStream_t s);
 int launch_spline_y2(const float *rf, int64_t nch, int32_t nt, float dt, float *uscr, float *y2,
                      hipStream_t s);
+// Samples per lane (32 or 64) of the one-wave-per-channel spline sweeps for
+// traces of nt samples, or 0 for the one-lane-per-channel sweeps (nt > 4096,
+// or PYCBF_SPLINE_SEG=0 in the environment at the call).
+int spline_seg_len(int32_t nt);
 int launch_upsample(const float *rf, const float *y2, int64_t nch, int32_t nt, float t0, float dt,
                     int32_t usf, float *out, hipStream_t s);
 int launch_korder_coefs(const float *rf, int64_t nch, int32_t nt, const float *S, int k,

@@ -21,7 +21,7 @@
 // bins that cover the tile's sample window (k_tile_bounds), and flushes the
 // bins to HBM with one float atomic per non-zero bin.  For the natural
 // cubic the bins hold the cotangents of the samples y and of the table's
-// e = dt^2/6 y2, and k_spline_adjoint adds the transposed spline solve
+// e = dt^2/6 y2, and k_spline_adjoint(_seg) adds the transposed spline solve
 //   dy += D2(T^-1 de),  T = tridiag(1, 4, 1),  D2 the second difference,
 // which is the adjoint of e = T^-1 D2 y (the natural spline with y2 = 0 at
 // both ends).
@@ -354,6 +354,192 @@ __global__ __launch_bounds__(64) void k_spline_adjoint(const float *__restrict__
     }
 }
 
+// Same solve for n <= 64*SEG, one wave per channel: lane l sweeps the SEG
+// samples i0 = l*SEG .. of the channel in registers, starting each sweep
+// from a guessed carry 32 samples outside its segment (both recurrences
+// contract by c'_inf = 2 - sqrt(3) per step) that is checked bit for bit
+// against the neighbouring lane's value and recomputed on a mismatch, as in
+// the spline prefilter: the result equals the one-lane-per-channel sweep.
+// Global traffic: ge and gy read once, drf written once, in coalesced
+// 64 x 32 tiles through one LDS tile.
+template <int SEG>
+__device__ __forceinline__ float adj_forward(const float (&e)[SEG], float dp, int i0, int n,
+                                             float cinf, const float (&clo)[CP_EXACT],
+                                             bool lane0, float (&v)[SEG])
+{
+#pragma unroll
+    for (int c = 0; c < SEG; ++c) {
+        const float ci = (c < CP_EXACT && lane0) ? clo[c < CP_EXACT ? c : 0] : cinf;
+        float dn = (e[c] - dp) * ci;
+        if (c == 0) dn = lane0 ? 0.0f : dn;  // d'_0 = 0
+        dp = dn;
+        v[c] = (i0 + c <= n - 2) ? dp : 0.0f;
+    }
+    return dp;
+}
+
+template <int SEG>
+__device__ __forceinline__ float adj_backward(float (&v)[SEG], float un, float cinf,
+                                              const float (&clo)[CP_EXACT], bool lane0)
+{
+#pragma unroll
+    for (int c = SEG - 1; c >= 0; --c) {
+        const float ci = (c < CP_EXACT && lane0) ? clo[c < CP_EXACT ? c : 0] : cinf;
+        float u = v[c] - ci * un;
+        if (c == 0) u = lane0 ? 0.0f : u;  // u_0 = 0
+        un = u;
+        v[c] = u;
+    }
+    return un;
+}
+
+template <int SEG>
+__global__ __launch_bounds__(64) void k_spline_adjoint_seg(const float *__restrict__ gy,
+                                                           const float *__restrict__ ge,
+                                                           int gstride, int n,
+                                                           float *__restrict__ drf)
+{
+    static_assert(SEG % 32 == 0, "segments are whole 32-sample tiles");
+    __shared__ __align__(16) SplRow S[64];
+    const int lane = threadIdx.x;
+    const bool lane0 = lane == 0;
+    const int64_t ch = blockIdx.x;
+    const float *erow = ge + ch * (int64_t)gstride;
+    const float *yrow = gy + ch * (int64_t)gstride;
+    float *orow = drf + ch * (int64_t)n;
+    const int i0 = lane * SEG;
+    const int c4 = 4 * (lane & 7);
+
+    // c'_i for i < CP_EXACT (c'_0 unused), c'_inf beyond
+    float clo[CP_EXACT];
+    {
+        float q = 0.25f;
+        clo[0] = 0.0f;
+        clo[1] = q;
+#pragma unroll
+        for (int i = 2; i < CP_EXACT; ++i) {
+            q = 1.0f / (4.0f - q);
+            clo[i] = q;
+        }
+    }
+    const float cinf = elim_factor(CP_EXACT);
+
+    // ---- load e: tile t = samples 32t .. 32t+31 of every segment ----------
+    float e[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG / 32; ++t) {
+        TileRegs r;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = (8 * q + (lane >> 3)) * SEG + 32 * t + c4;
+            F4u w{{0.0f, 0.0f, 0.0f, 0.0f}};
+            if (k + 4 <= n) {
+                w = *reinterpret_cast<const F4u *>(erow + k);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (k + j < n) w.v[j] = erow[k + j];
+            }
+            r.q[q] = w;
+        }
+        if (t > 0) __syncthreads();
+        tile_to_lds(r, S);
+        __syncthreads();
+        float part[32];
+        row_read(S, part);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) e[32 * t + j] = part[j];
+    }
+
+    // ---- forward elimination d'_i = (e_i - d'_{i-1}) c'_i -----------------
+    // guess of d'_{i0-1}: d' = 0 at i0-32, then the left neighbour's last 31
+    // e values (still in the LDS tile)
+    float carry = 0.0f;
+    if (lane > 0) {
+        const bool low = SEG == 32 && lane == 1;  // reaches i < CP_EXACT
+        float dp = 0.0f;
+#pragma unroll
+        for (int c = 1; c < 32; ++c) {
+            const float ci = (c < CP_EXACT && low) ? clo[c < CP_EXACT ? c : 0] : cinf;
+            dp = (S[lane - 1][c] - dp) * ci;
+        }
+        carry = dp;
+    }
+    float v[SEG];
+    float last = adj_forward<SEG>(e, carry, i0, n, cinf, clo, lane0, v);
+    for (;;) {
+        const float want = __shfl_up(last, 1);
+        const bool redo = lane > 0 && i0 <= n - 2 &&
+                          __float_as_uint(carry) != __float_as_uint(want);
+        if (__ballot(redo) == 0) break;
+        if (redo) {
+            carry = want;
+            last = adj_forward<SEG>(e, carry, i0, n, cinf, clo, lane0, v);
+        }
+    }
+    const float carry_fwd = carry;
+
+    // ---- back substitution u_i = d'_i - c'_i u_{i+1} ----------------------
+    __syncthreads();  // neighbours' e rows read
+#pragma unroll
+    for (int j = 0; j < 32; j += 4)  // d'_{i0 .. i0+31}
+        *reinterpret_cast<float4 *>(&S[lane][j]) = make_float4(v[j], v[j + 1], v[j + 2], v[j + 3]);
+    __syncthreads();
+    float bcarry = 0.0f;  // lane 63: u beyond the channel is 0
+    if (lane < 63) {
+        float un = 0.0f;
+#pragma unroll
+        for (int c = 31; c >= 0; --c) un = S[lane + 1][c] - cinf * un;
+        bcarry = un;
+    }
+    float first = adj_backward<SEG>(v, bcarry, cinf, clo, lane0);
+    for (;;) {
+        const float want = __shfl_down(first, 1);
+        const bool redo = lane < 63 && i0 < n && __float_as_uint(bcarry) != __float_as_uint(want);
+        if (__ballot(redo) == 0) break;
+        if (redo) {
+            bcarry = want;
+            adj_forward<SEG>(e, carry_fwd, i0, n, cinf, clo, lane0, v);
+            first = adj_backward<SEG>(v, bcarry, cinf, clo, lane0);
+        }
+    }
+
+    // ---- output o[k] = y[k] + (u_{k-1} - 2 u_k + u_{k+1}) ------------------
+    const float ul = __shfl_up(v[SEG - 1], 1);  // u_{i0-1}
+    const float ur = __shfl_down(first, 1);     // u_{i0+SEG}
+#pragma unroll
+    for (int c = 0; c < SEG; ++c) {
+        const float um = c > 0 ? v[c - 1] : (lane0 ? 0.0f : ul);
+        const float up = c + 1 < SEG ? v[c + 1] : (lane == 63 ? 0.0f : ur);
+        e[c] = um - 2.0f * v[c] + up;  // e is free: holds the second difference
+    }
+#pragma unroll
+    for (int t = 0; t < SEG / 32; ++t) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 32; j += 4)
+            *reinterpret_cast<float4 *>(&S[lane][j]) =
+                make_float4(e[32 * t + j], e[32 * t + j + 1], e[32 * t + j + 2], e[32 * t + j + 3]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int rr = 8 * q + (lane >> 3);
+            const int k = rr * SEG + 32 * t + c4;
+            const float4 w = *reinterpret_cast<const float4 *>(&S[rr][c4]);
+            const float d4[4] = {w.x, w.y, w.z, w.w};
+            if (k + 4 <= n) {
+                const F4u y = *reinterpret_cast<const F4u *>(yrow + k);
+                F4u o{{y.v[0] + d4[0], y.v[1] + d4[1], y.v[2] + d4[2], y.v[3] + d4[3]}};
+                *reinterpret_cast<F4u *>(orow + k) = o;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (k + j < n) orow[k + j] = yrow[k + j] + d4[j];
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill_zero(float *__restrict__ p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -401,8 +587,16 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         if (int rc = check_launch("das_bwd_rf")) return rc;
         if (GK == GK_CUBIC) {
             if (a.nt > SPL_MAX_NT) return fail(PYCBF_EINVAL, "spline adjoint: nt too large");
-            hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0,
-                               s, a.gy, a.ge, a.gstride, nch, a.nt, grf);
+            const int seg = spline_seg_len(a.nt);
+            if (seg == 32)
+                hipLaunchKernelGGL(k_spline_adjoint_seg<32>, dim3((unsigned)nch), dim3(64), 0, s,
+                                   a.gy, a.ge, a.gstride, a.nt, grf);
+            else if (seg == 64)
+                hipLaunchKernelGGL(k_spline_adjoint_seg<64>, dim3((unsigned)nch), dim3(64), 0, s,
+                                   a.gy, a.ge, a.gstride, a.nt, grf);
+            else
+                hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 63) / 64)), dim3(64),
+                                   0, s, a.gy, a.ge, a.gstride, nch, a.nt, grf);
             if (int rc = check_launch("spline_adjoint")) return rc;
         }
     }

@@ -17,6 +17,7 @@
 //    derivatives in consistent per-sample units (the reference mixes units,
 //    pycbf/gpu/__engines__.cu:153-160,261-268).
 #include <math.h>
+#include <stdlib.h>
 
 #include "common.h"
 #include "prof.h"
@@ -499,9 +500,10 @@ __global__ __launch_bounds__(64) void k_spline_seg(const float *__restrict__ rf,
     }
 }
 
-// Segment length for the one-wave-per-channel kernel, 0 when nt is too long.
-static int spline_seg(int32_t nt)
+int spline_seg_len(int32_t nt)
 {
+    if (const char *e = getenv("PYCBF_SPLINE_SEG"))
+        if (atoi(e) == 0) return 0;
     if (nt <= 64 * 32) return 32;
     if (nt <= 64 * 64) return 64;
     return 0;
@@ -513,7 +515,7 @@ static int launch_spline(const float *rf, int64_t nch, int32_t nt, float dt, flo
 {
     if (nch <= 0 || nt <= 0) return 0;
     if (nt > SPL_MAX_NT) return fail(1, "spline: nt %d exceeds %d samples", nt, SPL_MAX_NT);
-    switch (spline_seg(nt)) {
+    switch (spline_seg_len(nt)) {
     case 32:
         hipLaunchKernelGGL((k_spline_seg<MODE, 32>), dim3((unsigned)nch), dim3(64), 0, s, rf,
                            (int)nt, dt, out);

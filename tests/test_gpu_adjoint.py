@@ -9,6 +9,8 @@ gradients within 1e-3 relative of FP64 central differences; the channel-data
 adjoint satisfies <A x, y> = <x, A^T y> to 1e-5 relative at the
 differentiable workload's full size (31 tx x 128 rx x 4096 samples, 256 x 256
 pixels)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -117,11 +119,13 @@ def test_delay_gradient_matches_fp64_central_differences():
             assert abs(g[ai][idx] - fd) <= 1e-3 * max(abs(fd), 1e-2 * peak), (ai, idx, g[ai][idx], fd)
 
 
-@pytest.mark.parametrize("nt", [1, 2, 3, 31, 33, 64, 100, 517])
+@pytest.mark.parametrize("nt", [1, 2, 3, 31, 33, 64, 100, 517, 2047, 2049, 4096, 4100])
 def test_cubic_rf_adjoint_trace_lengths(nt):
-    """The transposed spline solve (tiled over 64 channels x 32 samples) for
-    trace lengths around the tile edges, over 70 channels (a ragged second
-    wave): channel-data gradients against FP64 autograd."""
+    """The transposed spline solve for trace lengths around the segment and
+    tile edges, over 70 channels: up to 4096 samples one wave per channel
+    (lane segments from checked carries; the sparse cotangent bins of 120
+    pixels leave long zero runs that force recomputed carries), beyond that
+    one lane per channel.  Channel-data gradients against FP64 autograd."""
     print(f"cubic rf adjoint nt={nt}", flush=True)
     arrs = list(_case(40 + nt, ntx=5, nrx=14, nt=nt, np_=120)[:5])
     t0, dt = 1e-6, 2.5e-8
@@ -130,7 +134,21 @@ def test_cubic_rf_adjoint_trace_lengths(nt):
     _, g = _grads_gpu(arrs, t0, dt, "cubic", "tx_and_rx", 0.0, gout)
     _, r = _grads_ref(arrs, t0, dt, "cubic", "tx_and_rx", 0.0, gout)
     peak = max(float(np.abs(r[0]).max()), 1e-30)
-    assert float(np.abs(g[0] - r[0]).max()) <= 1e-4 * peak
+    # float32 sample positions x <= t0 + nt*dt carry a rounding of 2^-23 x,
+    # a relative error of 2^-23 x / dt in the interpolation weights that the
+    # FP64 reference does not have (it reaches 1e-4 near 850 samples here)
+    tol = max(1e-4, 2.0 ** -23 * (t0 / dt + nt))
+    assert float(np.abs(g[0] - r[0]).max()) <= tol * peak
+    if nt > 64:
+        # the segment sweeps agree with the one-lane-per-channel sweeps; the
+        # cotangent bins come from float atomics in varying order, so two runs
+        # differ in the last bits whichever solve follows
+        os.environ["PYCBF_SPLINE_SEG"] = "0"
+        try:
+            _, g1 = _grads_gpu(arrs, t0, dt, "cubic", "tx_and_rx", 0.0, gout)
+        finally:
+            del os.environ["PYCBF_SPLINE_SEG"]
+        assert float(np.abs(g[0] - g1[0]).max()) <= 1e-6 * peak
 
 
 @pytest.mark.parametrize("kind", ["nearest", "linear", "cubic"])

@@ -4,6 +4,7 @@ Bars: identical sample-index selection (bitwise), per-sample outputs of
 nearest / linear bitwise equal to the oracle, images within 1e-5 of the
 oracle's image peak (float32 sums in a different order)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -120,6 +121,15 @@ def test_spline_prefilter_bitwise(nt, data):
     ref = np.stack([oracle.spline_y2(r, float(dt)) for r in rf])
     got = y2.cpu().numpy()
     assert np.array_equal(got, ref), f"{int((got != ref).sum())} y2 values differ"
+    if 64 < nt <= 4096:
+        # the one-lane-per-channel sweep (PYCBF_SPLINE_SEG=0) gives the same bits
+        os.environ["PYCBF_SPLINE_SEG"] = "0"
+        try:
+            E.check(E.lib.pycbf_spline_y2(E.ptr(x), nch, nt, float(dt), E.ptr(y2), E.ptr(ws), wsb,
+                                          s), "y2")
+            assert np.array_equal(y2.cpu().numpy(), ref)
+        finally:
+            del os.environ["PYCBF_SPLINE_SEG"]
     h = np.float32(np.float32(1.0) / np.float32(6.0)) * (dt * dt)
     t = tab.cpu().numpy()
     assert np.array_equal(t[:, :nt, 0], rf)
