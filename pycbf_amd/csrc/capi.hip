@@ -482,7 +482,7 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 
 // ------------------------------------------------------------- backward ---
 struct BwdPlan {
-    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, total = 0;
+    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, mbits = 256, total = 0;
 };
 
 static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind)
@@ -496,7 +496,7 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
         b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
     }
-    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + 5 * 256;
+    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.mbits + 6 * 256;
     return b;
 }
 
@@ -553,6 +553,7 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
     b.gout = gout;
     float *bounds = (float *)cv.take(plan.bounds);
     b.bounds = bounds;
+    b.mbits = (unsigned *)cv.take(plan.mbits);
     if (interp_kind == PYCBF_INTERP_CUBIC) {
         float2 *coef = (float2 *)cv.take(plan.coef);
         float *u = (float *)cv.take(plan.uscr);

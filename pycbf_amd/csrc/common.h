@@ -128,6 +128,7 @@ struct BwdArgs {
     float *gtau, *gapod;  // outer-axis gradients (may be null)
     float *gy, *ge;       // channel-data cotangent bins (ge: cubic only)
     int gstride;          // floats per channel row of gy / ge
+    unsigned *mbits;      // 3 words of workspace: max |gout|, |apodtx|, |apodrx| (bits)
 };
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
                    float *gapodrx, float *grf, hipStream_t s);

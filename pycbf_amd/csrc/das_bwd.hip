@@ -15,17 +15,21 @@
 //   d taurx[rx, p]  = sum_f sum_tx G w v'(x)     d apodrx[rx, p] = sum_f sum_tx G apodtx v(x)
 //   d rf[f, tx, rx, k] = sum_p G w dv(x)/dy_k                       (transposed gather)
 // k_das_bwd_tau evaluates the first four with one lane per (element, pixel),
-// recomputing x and w from the tables.  k_das_bwd_rf scatters the
-// channel-data gradient: one workgroup per (pixel tile, transmit, frame)
-// walks the receive channels, accumulates the tile's contributions in LDS
-// bins that cover the tile's sample window (k_tile_bounds), and flushes the
-// bins to HBM with one float atomic per non-zero bin.  For the natural
+// recomputing x and w from the tables.  The channel-data gradient is a
+// scatter into 64-bit fixed-point LDS bins (scale from k_bwd_scale's bound):
+// k_das_bwd_rf_ch (traces up to 4096 samples) gives each workgroup one
+// channel and writes its rows once; k_das_bwd_rf (longer traces) gives each
+// workgroup a (pixel tile, transmit, frame), bins the tile's sample window
+// per receive channel (k_tile_bounds) and flushes it with one float atomic
+// per non-zero bin.  For the natural
 // cubic the bins hold the cotangents of the samples y and of the table's
 // e = dt^2/6 y2, and k_spline_adjoint(_seg) adds the transposed spline solve
 //   dy += D2(T^-1 de),  T = tridiag(1, 4, 1),  D2 the second difference,
 // which is the adjoint of e = T^-1 D2 y (the natural spline with y2 = 0 at
 // both ends).
 #include <math.h>
+
+#include <algorithm>
 
 #include "common.h"
 #include "das_dev.h"
@@ -127,12 +131,73 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
     if (A.gapod) A.gapod[(int64_t)o * A.np + p] = ga;
 }
 
-// transposed gather: one workgroup per (pixel tile, transmit, frame)
+// max |value| of gout, apodtx and apodrx (blockIdx.y = 0, 1, 2), as bit
+// patterns in mbits[blockIdx.y] (non-negative floats order like their bits;
+// a NaN or an infinity records the infinity pattern).  |gout * apodtx *
+// apodrx| <= m0 m1 m2 bounds every contribution of the transposed gather.
+__global__ __launch_bounds__(256) void k_bwd_scale(BwdArgs A, int64_t ngout)
+{
+    const int which = blockIdx.y;
+    const float *x = which == 0 ? A.gout : (which == 1 ? A.atx : A.arx);
+    const int64_t n = which == 0 ? ngout : (int64_t)(which == 1 ? A.ntx : A.nrx) * A.np;
+    auto fold = [](float m, float v) {
+        v = fabsf(v);
+        return fmaxf(m, (v <= 3.0e38f) ? v : __int_as_float(0x7f800000));
+    };
+    float m = 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t head = 0;  // scalar elements before the first 16-byte boundary
+    if (reinterpret_cast<uintptr_t>(x) & 15)
+        head = min((int64_t)((16 - (reinterpret_cast<uintptr_t>(x) & 15)) / 4), n);
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < head; q += stride)
+        m = fold(m, x[q]);
+    const float4 *x4 = reinterpret_cast<const float4 *>(x + head);
+    const int64_t n4 = (n - head) / 4;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n4; q += stride) {
+        const float4 v = x4[q];
+        m = fold(fold(fold(fold(m, v.x), v.y), v.z), v.w);
+    }
+    for (int64_t q = head + 4 * n4 + (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride)
+        m = fold(m, x[q]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    // one atomic per workgroup: atomics on one word serialise in L2
+    __shared__ float wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(A.mbits + which,
+                  __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
+}
+
+// Fixed-point scale of the LDS bins, or 0 when a bound is not finite (the
+// bins then take float atomics).  A bin receives at most 2^per_bin
+// contributions, each below M (1 + 2^-22) with M = m0 m1 m2; the scale
+// 2^(59 - per_bin - ilogb M) keeps every bin below 2^61 and quantises a
+// contribution to 2^(per_bin - 59) M.
+// per_bin: log2 of the most contributions one bin can receive (9 for the
+// 2 x 256 of a pixel tile).
+__device__ __forceinline__ double bwd_scale(const unsigned *mbits, int per_bin = 9)
+{
+    const float m0 = __uint_as_float(mbits[0]), m1 = __uint_as_float(mbits[1]),
+                m2 = __uint_as_float(mbits[2]);
+    if (!(m0 <= 3.0e38f && m1 <= 3.0e38f && m2 <= 3.0e38f)) return 0.0;
+    const double M = (double)m0 * (double)m1 * (double)m2;
+    if (!(M > 0.0)) return 1.0;  // every contribution is zero
+    return ldexp(1.0, 59 - per_bin - ilogb(M));
+}
+
+// transposed gather: one workgroup per (pixel tile, transmit, frame).  The
+// LDS bins are 64-bit fixed-point integers (ds_add_u64): on gfx950 integer
+// LDS atomics run at about 37x the rate of float LDS atomics, and integer
+// sums do not depend on the order of the additions.
 template <int GK>
 __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
 {
-    __shared__ float by[BWD_BINS];
-    __shared__ float be[GK == GK_CUBIC ? BWD_BINS : 1];
+    __shared__ unsigned long long by[BWD_BINS];
+    __shared__ unsigned long long be[GK == GK_CUBIC ? BWD_BINS : 1];
+    float *byf = reinterpret_cast<float *>(by);  // float bins (bound not finite)
+    float *bef = reinterpret_cast<float *>(be);
     const int64_t tile = blockIdx.x;
     const int tx = blockIdx.y, f = blockIdx.z;
     const int64_t p = tile * 256 + threadIdx.x;
@@ -142,9 +207,29 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
     const float txlo = bt[0], txhi = bt[1];
     const float ttx = act ? A.ttx[(int64_t)tx * A.np + p] : 0.0f;
     const float atx = act ? A.atx[(int64_t)tx * A.np + p] : 0.0f;
+    const double scale = bwd_scale(A.mbits);
+    const bool fix = scale > 0.0;  // uniform
+    const double iscale = fix ? 1.0 / scale : 0.0;
+    auto bin_add = [&](unsigned long long *bins, float *binsf, int i, float v) {
+        if (fix)
+            atomicAdd(bins + i, (unsigned long long)(long long)((double)v * scale));
+        else
+            atomicAdd(binsf + i, v);
+    };
+    auto bin_take = [&](unsigned long long *bins, float *binsf, int i) -> float {
+        float v;
+        if (fix) {
+            v = (float)((double)(long long)bins[i] * iscale);
+            bins[i] = 0ull;
+        } else {
+            v = binsf[i];
+            binsf[i] = 0.0f;
+        }
+        return v;
+    };
     for (int t = threadIdx.x; t < BWD_BINS; t += 256) {
-        by[t] = 0.0f;
-        if (GK == GK_CUBIC) be[t] = 0.0f;
+        by[t] = 0ull;
+        if (GK == GK_CUBIC) be[t] = 0ull;
     }
     __syncthreads();
     for (int rx = 0; rx < A.nrx; ++rx) {
@@ -180,7 +265,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                 if (GK == GK_NEAREST) {
                     int it = 0;
                     if (nearest_index(x, A.t0, A.dt, A.nt, it) && g != 0.0f) {
-                        if (binned) atomicAdd(&by[it - lo], g);
+                        if (binned) bin_add(by, byf, it - lo, g);
                         else unsafeAtomicAdd(gy + it, g);
                     }
                 } else {
@@ -200,8 +285,8 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                         // linear the forward reads 0 past the last sample
                         const bool up = (GK == GK_CUBIC) || (j + 1 < A.nt);
                         if (binned) {
-                            atomicAdd(&by[j - lo], g * a);
-                            if (up) atomicAdd(&by[j + 1 - lo], g * b);
+                            bin_add(by, byf, j - lo, g * a);
+                            if (up) bin_add(by, byf, j + 1 - lo, g * b);
                         } else {
                             unsafeAtomicAdd(gy + j, g * a);
                             if (up) unsafeAtomicAdd(gy + j + 1, g * b);
@@ -210,8 +295,8 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                             const float ca = __builtin_fmaf(a * a, a, -a);
                             const float cb = __builtin_fmaf(b * b, b, -b);
                             if (binned) {
-                                atomicAdd(&be[j - lo], g * ca);
-                                atomicAdd(&be[j + 1 - lo], g * cb);
+                                bin_add(be, bef, j - lo, g * ca);
+                                bin_add(be, bef, j + 1 - lo, g * cb);
                             } else {
                                 unsafeAtomicAdd(ge + j, g * ca);
                                 unsafeAtomicAdd(ge + j + 1, g * cb);
@@ -225,16 +310,106 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
         __syncthreads();
         // flush the window and clear the bins for the next channel
         for (int t = threadIdx.x; t < cnt; t += 256) {
-            const float vy = by[t];
-            by[t] = 0.0f;
+            const float vy = bin_take(by, byf, t);
             if (vy != 0.0f && lo + t < A.gstride) unsafeAtomicAdd(gy + lo + t, vy);
             if (GK == GK_CUBIC) {
-                const float ve = be[t];
-                be[t] = 0.0f;
+                const float ve = bin_take(be, bef, t);
                 if (ve != 0.0f) unsafeAtomicAdd(ge + lo + t, ve);
             }
         }
         __syncthreads();
+    }
+}
+
+// Transposed gather with one workgroup per channel (frame, transmit,
+// receive) for traces of up to 4096 samples: the workgroup sweeps every pixel
+// of the channel into fixed-point LDS bins covering the whole trace (y and,
+// for the cubic, e = dt^2/6 y2: 2 x 4097 x 8 bytes) and writes the channel's
+// cotangent rows once with plain stores.  No global atomics, no zero fill of
+// the rows beforehand, and the integer sums make the result independent of
+// the order of the pixels.  16 waves per workgroup sweep 1024 pixels per
+// step; a wave skips its 64 pixels when the tile's delay range leaves the
+// channel unused.
+template <int GK>
+__global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
+{
+    extern __shared__ unsigned long long bins[];
+    const int nb = A.gstride;  // bins per array: the row length
+    unsigned long long *by = bins;
+    unsigned long long *be = bins + nb;  // cubic only
+    float *byf = reinterpret_cast<float *>(by);
+    float *bef = reinterpret_cast<float *>(be);
+    const int tid = threadIdx.x;
+    const int64_t chan = blockIdx.x;
+    const int rx = (int)(chan % A.nrx);
+    const int tx = (int)((chan / A.nrx) % A.ntx);
+    const int f = (int)(chan / ((int64_t)A.nrx * A.ntx));
+    const int nent = A.ntx + A.nrx;
+    // log2 of the contributions a bin can take: 2 per pixel
+    const int per_bin = 64 - __clzll((unsigned long long)(2 * A.np));
+    const double scale = bwd_scale(A.mbits, per_bin);
+    const bool fix = scale > 0.0;  // uniform
+    const double iscale = fix ? 1.0 / scale : 0.0;
+    auto bin_add = [&](unsigned long long *b, float *bf, int i, float v) {
+        if (fix)
+            atomicAdd(b + i, (unsigned long long)(long long)((double)v * scale));
+        else
+            atomicAdd(bf + i, v);
+    };
+    const int narr = GK == GK_CUBIC ? 2 : 1;
+    for (int t = tid; t < narr * nb; t += blockDim.x) bins[t] = 0ull;
+    __syncthreads();
+
+    const float *ttx = A.ttx + (int64_t)tx * A.np, *atx = A.atx + (int64_t)tx * A.np;
+    const float *trx = A.trx + (int64_t)rx * A.np, *arx = A.arx + (int64_t)rx * A.np;
+    for (int64_t base = 0; base < A.np; base += blockDim.x) {
+        const int64_t p = base + tid;
+        // the wave's pixels lie in one tile of 256: skip it when the tile's
+        // delay range selects no sample of this channel (wave-uniform)
+        const int64_t tile = (base + (tid & ~63)) >> 8;
+        if ((base + (tid & ~63)) >= A.np) continue;
+        const float *bt = A.bounds + (tile * nent + tx) * 2;
+        const float *br = A.bounds + (tile * nent + A.ntx + rx) * 2;
+        const float xlo = bt[0] + br[0], xhi = bt[1] + br[1];
+        if (!(xlo <= xhi)) continue;
+        if (p >= A.np) continue;
+        const float w = atx[p] * arx[p];
+        if (!(fabsf(w) >= A.thresh)) continue;
+        const float x = ttx[p] + trx[p];
+        const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)] * w;
+        if (g == 0.0f) continue;
+        if (GK == GK_NEAREST) {
+            int it = 0;
+            if (nearest_index(x, A.t0, A.dt, A.nt, it)) bin_add(by, byf, it, g);
+        } else {
+            int j = 0;
+            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
+            const float xlead = A.t0 + A.dt * (float)(j + 1);
+            const float xlag = A.t0 + A.dt * (float)j;
+            float a, b;
+            if (GK == GK_LINEAR) {
+                a = (xlead - x) / A.dt;
+                b = (x - xlag) / A.dt;
+            } else {
+                a = (xlead - x) * A.rdt;
+                b = (x - xlag) * A.rdt;
+            }
+            // sample j+1 exists for the cubic table (zero entry at nt); for
+            // linear the forward reads 0 past the last sample
+            bin_add(by, byf, j, g * a);
+            if (GK == GK_CUBIC || j + 1 < A.nt) bin_add(by, byf, j + 1, g * b);
+            if (GK == GK_CUBIC) {
+                bin_add(be, bef, j, g * __builtin_fmaf(a * a, a, -a));
+                bin_add(be, bef, j + 1, g * __builtin_fmaf(b * b, b, -b));
+            }
+        }
+    }
+    __syncthreads();
+    float *gy = A.gy + chan * A.gstride;
+    float *ge = GK == GK_CUBIC ? A.ge + chan * A.gstride : nullptr;
+    for (int t = tid; t < nb; t += blockDim.x) {
+        gy[t] = fix ? (float)((double)(long long)by[t] * iscale) : byf[t];
+        if (GK == GK_CUBIC) ge[t] = fix ? (float)((double)(long long)be[t] * iscale) : bef[t];
     }
 }
 
@@ -577,13 +752,36 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
     if (grf) {
         const int64_t nch = (int64_t)a.nfr * a.ntx * a.nrx;
         ProfScope ps("das_bwd_rf", s);
-        if (GK == GK_CUBIC) {
-            if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
-            if (int rc = fill_zero(a.ge, nch * a.gstride, s)) return rc;
+        // one workgroup per channel while the whole trace's bins fit in LDS
+        const size_t ch_lds = (size_t)(GK == GK_CUBIC ? 2 : 1) * a.gstride * 8;
+        int dev = 0, max_lds = 0;
+        if (int rc = check(hipGetDevice(&dev), "device")) return rc;
+        if (int rc = check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                 dev),
+                           "LDS per workgroup"))
+            return rc;
+        const bool per_channel = a.gstride <= 4097 && ch_lds <= (size_t)max_lds && nch <= 0x7fffffff;
+        // bound of the contributions for the fixed-point bins
+        if (int rc = fill_zero(reinterpret_cast<float *>(a.mbits), 3, s)) return rc;
+        int64_t ngout = (int64_t)a.nfr * a.np;
+        if (a.sumflag == PYCBF_SUM_NONE) ngout *= (int64_t)a.ntx * a.nrx;
+        else if (a.sumflag == PYCBF_SUM_TX_ONLY) ngout *= a.nrx;
+        else if (a.sumflag == PYCBF_SUM_RX_ONLY) ngout *= a.ntx;
+        const int64_t nscan = std::max<int64_t>(ngout, (int64_t)std::max(a.ntx, a.nrx) * a.np);
+        const unsigned sb = (unsigned)std::min<int64_t>(256, (nscan / 4 + 255) / 256 + 1);
+        hipLaunchKernelGGL(k_bwd_scale, dim3(sb, 3), dim3(256), 0, s, a, ngout);
+        if (int rc = check_launch("das_bwd_scale")) return rc;
+        if (per_channel) {
+            hipLaunchKernelGGL(k_das_bwd_rf_ch<GK>, dim3((unsigned)nch), dim3(1024), ch_lds, s, a);
         } else {
-            if (int rc = fill_zero(grf, nch * a.nt, s)) return rc;
+            if (GK == GK_CUBIC) {
+                if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
+                if (int rc = fill_zero(a.ge, nch * a.gstride, s)) return rc;
+            } else {
+                if (int rc = fill_zero(grf, nch * a.nt, s)) return rc;
+            }
+            hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
         }
-        hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
         if (int rc = check_launch("das_bwd_rf")) return rc;
         if (GK == GK_CUBIC) {
             if (a.nt > SPL_MAX_NT) return fail(PYCBF_EINVAL, "spline adjoint: nt too large");
