@@ -304,6 +304,60 @@ def sos_leg(dev, steps):
     }
 
 
+def adjoint_leg(dev, reps):
+    """Channel-data adjoint: the cubic Tabbed DAS of 31 plane waves over
+    +-15 deg x 128 rx x 4096 samples onto 256 x 256 pixels (the
+    differentiable workload's shape), forward and backward with respect to the
+    channel data through the autograd Function; seeded Gaussian channel data
+    and upstream gradient.  Returns the JSON object of the leg."""
+    import torch
+
+    from pycbf_amd import sim
+    from pycbf_amd.gpu import _engine as E
+    from pycbf_amd.gpu.autograd import das_tabbed
+    ntx, nrx, nt, side = 31, 128, 4096, 256
+    xele = sim.linear_array(nrx)
+    steers = np.radians(np.linspace(-15, 15, ntx))
+    _, _, pnts = sim.grid(side, side, (-19e-3, 19e-3), (5e-3, 45e-3))
+    tabs = [torch.as_tensor(a, device=dev) for a in sim.pw_tabs(steers, xele, pnts)]
+    g = torch.Generator(device=dev)
+    g.manual_seed(18)
+    rf = torch.randn((ntx, nrx, nt), generator=g, device=dev, requires_grad=True)
+    G = torch.randn((side * side,), generator=g, device=dev)
+
+    def step():
+        out = das_tabbed(rf, *tabs, 0.0, 1.0 / sim.FS, "cubic", "tx_and_rx", 0.0)
+        (grf,) = torch.autograd.grad((out * G).sum(), [rf])
+        return grf
+
+    step()
+    torch.cuda.synchronize()
+    E.lib.pycbf_profile_reset()
+    E.lib.pycbf_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    E.lib.pycbf_profile_enable(0)
+    kern = {}
+    for k in ("das_gather", "das_bwd_rf", "spline_coefs"):
+        ms, n = E.profile_read(k)
+        kern[k] = round(ms / max(n, 1), 4)
+    samples = ntx * nrx * side * side
+    log(f"adjoint leg: forward + channel-data backward {wall * 1e3:.3f} ms, "
+        f"das_bwd_rf {kern['das_bwd_rf']:.3f} ms")
+    del rf, tabs
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"cubic Tabbed DAS {ntx} PW x {nrx} rx x {nt} samples onto {side}x{side} px, "
+                    "forward + backward with respect to the channel data",
+        "ms_fwd_bwd": round(wall * 1e3, 3),
+        "fwd_bwd_gsamples_per_s": round(samples / wall / 1e9, 2),
+        "kernel_ms": kern,
+    }
+
+
 def batch_leg(dev, runs, speckle):
     """Ragged batch: 64 seeded requests, grid side in {128, 192, 256, 384, 512},
     Ntx in {1, 11, 31, 75}, Nrx in {64, 128, 192, 256}, Nt = 4096; cubic DAS
@@ -387,6 +441,8 @@ def main():
     ap.add_argument("--batch-runs", type=int, default=5)
     ap.add_argument("--batch-speckle", type=int, default=2000)
     ap.add_argument("--slsc-reps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the channel-data adjoint leg")
+    ap.add_argument("--adjoint-reps", type=int, default=10)
     ap.add_argument("--data", choices=["sim", "noise"], default="sim",
                     help="noise: seeded Gaussian RF of the same shape (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -418,6 +474,10 @@ def main():
         n = args.calib_copy_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
+        # evict the freshly written source from L2 and the infinity cache, so
+        # that the copy's reads all come from HBM
+        flush = torch.ones(2 * n, dtype=torch.float32, device=dev)
+        del flush
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
@@ -469,6 +529,7 @@ def main():
         del rf
         torch.cuda.empty_cache()
         slsc_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
+    adj_res = None if args.no_adjoint else adjoint_leg(dev, args.adjoint_reps)
     sos_res = None if args.no_sos else sos_leg(dev, args.sos_steps)
     batch_res = None if args.no_batch else batch_leg(dev, args.batch_runs, args.batch_speckle)
 
@@ -498,6 +559,7 @@ def main():
                       "spline_coefs": round(s_ms / max(s_n, 1), 4),
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
         "slsc": slsc_res,
+        "adjoint": adj_res,
         "sos": sos_res,
         "batch": batch_res,
     }
