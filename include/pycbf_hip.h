@@ -135,6 +135,29 @@ int pycbf_synthetic_tables(const pycbf_rfinfo *info, const float *txpack, const 
                            float c0, const float *pvec, int64_t np, float *tautx, float *apodtx,
                            float *taurx, float *apodrx, void *stream);
 
+/* Delay tabs and apodisations of a 2-D linear array, written on the device:
+ * the per-pixel part of make_tabs_and_apods_2D
+ * (pycbf/helpers/__full_aperture_2D__.py:3-203), in float64 with the
+ * reference's operation order, rounded to float32.  Pixels p = ix*nz + iz of
+ * meshgrid(xout, zout, indexing='ij').  txc (device, ntx x 8 doubles) holds
+ * the per-transmit constants the host derives with the reference's NumPy
+ * calls:
+ *   PYCBF_TABS_PLANE      {ox, oz, nx, nz, nperp_x, nperp_z, xele[0], xele[-1]}
+ *                         (steering corrected for ctx, :150-160)
+ *   PYCBF_TABS_DIVERGENT  {ox, oz, tau0}  (:62-76)
+ *   PYCBF_TABS_FOCUSED    {ox, oz, nx, nz, t0, vstar}: the synthetic point of
+ *                         :270-314, and arccos(v) < ala  <=>  vstar <= v <= 1
+ * xele (nrx), xout (nx), zout (nz): device doubles.  Outputs (float32,
+ * device) tautx / apodtx (ntx, nx*nz), taurx / apodrx (nrx, nx*nz); any may
+ * be NULL.  Apodisations are 1.0 / 0.0 (the reference returns bool). */
+#define PYCBF_TABS_PLANE 0
+#define PYCBF_TABS_DIVERGENT 1
+#define PYCBF_TABS_FOCUSED 2
+int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, const double *xele,
+                       const double *xout, int64_t nx, const double *zout, int64_t nz, double cm,
+                       double fnum, float *tautx, float *apodtx, float *taurx, float *apodrx,
+                       void *stream);
+
 /* HOST function.  Spatially compact processing order for np pixel positions
  * (host pvec (np, ndim)) written to host perm (np).  No reference counterpart:
  * the reference launches one thread per sample in pixel-index order. */

@@ -319,6 +319,28 @@ int pycbf_das_tabbed(const pycbf_rfinfo *info, const float *rf, int64_t nfr, con
                    interp_param, S, thresh, sumflag, out, plan, cv, (hipStream_t)stream);
 }
 
+int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, const double *xele,
+                       const double *xout, int64_t nx, const double *zout, int64_t nz, double cm,
+                       double fnum, float *tautx, float *apodtx, float *taurx, float *apodrx,
+                       void *stream)
+{
+    if (mode != PYCBF_TABS_PLANE && mode != PYCBF_TABS_DIVERGENT && mode != PYCBF_TABS_FOCUSED)
+        return fail(PYCBF_EINVAL, "tabs mode %d is not plane (0), divergent (1) or focused (2)",
+                    mode);
+    if (ntx < 0 || nrx < 0 || ntx + nrx > 65535)
+        return fail(PYCBF_EINVAL, "need 0..65535 transmit plus receive elements");
+    if (nx < 0 || nz < 0) return fail(PYCBF_EINVAL, "negative grid size");
+    if (nx * nz > 0 && (!xout || !zout)) return fail(PYCBF_EINVAL, "xout / zout are NULL");
+    if (ntx > 0 && (tautx || apodtx) && !txc) return fail(PYCBF_EINVAL, "txc is NULL");
+    if (nrx > 0 && (taurx || apodrx) && !xele) return fail(PYCBF_EINVAL, "xele is NULL");
+    if (!(cm > 0.0)) return fail(PYCBF_EINVAL, "cm must be > 0");
+    const int nt = (tautx || apodtx) ? (int)ntx : 0;
+    const int nr = (taurx || apodrx) ? (int)nrx : 0;
+    // transmit rows first; with no transmit outputs the receive rows start at 0
+    return launch_tabs_2d(mode, nt, txc, nr, xele, xout, nx, zout, nz, cm, fnum, tautx, apodtx,
+                          taurx, apodrx, (hipStream_t)stream);
+}
+
 int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,
                          const float *t0tx, const float *alatx, const float *doftx, int64_t nrx,
                          const float *ovecrx, const float *nvecrx, const float *alarx,

@@ -133,6 +133,12 @@ struct BwdArgs {
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
                    float *gapodrx, float *grf, hipStream_t s);
 
+// ------------------------------------------------------------ delay tabs ---
+int launch_tabs_2d(int mode, int ntx, const double *txc, int nrx, const double *xele,
+                   const double *xout, int64_t nx, const double *zout, int64_t nz, double cm,
+                   double fnum, float *tautx, float *apodtx, float *taurx, float *apodrx,
+                   hipStream_t s);
+
 // ------------------------------------------------------------------ DMAS ---
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                 int nlag, int dmastype, int sumlags, float *out, hipStream_t s);

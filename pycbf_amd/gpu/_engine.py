@@ -33,6 +33,7 @@ lib = ct.CDLL(LIBPATH)
 INTERP = {"nearest": 0, "linear": 1, "akima": 2, "makima": 3, "korder_cubic": 4, "cubic": 5}
 SUMTYPE = {"none": 0, "tx_only": 1, "rx_only": 2, "tx_and_rx": 3}
 DMASTYPE = {"ssr": 0, "power": 1, "pssr": 2}
+TABS = {"plane": 0, "divergent": 1, "focused": 2}
 SLSC_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 TXPACK_FLOATS = 16
 RXPACK_FLOATS = 8
@@ -84,6 +85,8 @@ _SIGS = {
     "pycbf_synthetic_tables": ([_prf, _vp, _vp, _f32, _vp, _i64, _vp, _vp, _vp, _vp, _vp],
                                ct.c_int),
     "pycbf_tile_order": ([_vp, _i64, ct.c_int, _vp], ct.c_int),
+    "pycbf_make_tabs_2d": ([ct.c_int, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, ct.c_double,
+                            ct.c_double, _vp, _vp, _vp, _vp, _vp], ct.c_int),
     "pycbf_spline_coefs": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_y2": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_workspace_bytes": ([_i64, _i32], _sz),

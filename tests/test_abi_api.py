@@ -52,6 +52,8 @@ def test_header_constants_match_binding():
         assert consts["PYCBF_SUM_" + name.upper()] == code
     for name, code in E.DMASTYPE.items():
         assert consts["PYCBF_DMAS_" + name.upper()] == code
+    for name, code in E.TABS.items():
+        assert consts["PYCBF_TABS_" + name.upper()] == code
     assert consts["PYCBF_TXPACK_FLOATS"] == E.TXPACK_FLOATS
     assert consts["PYCBF_RXPACK_FLOATS"] == E.RXPACK_FLOATS
     assert ct.sizeof(E.RFInfo) == 32
@@ -96,6 +98,13 @@ def test_host_entry_points_report_argument_errors():
     assert b"ndim" in E.lib.pycbf_last_error()
     perm = np.empty(4, np.int32)
     assert E.lib.pycbf_tile_order(None, 4, 2, E.ptr(perm)) != 0
+    assert b"NULL" in E.lib.pycbf_last_error()
+    # the tab generator validates its arguments before any device work
+    assert E.lib.pycbf_make_tabs_2d(7, 1, None, 1, None, None, 1, None, 1, 1540.0, 1.5, None,
+                                    None, None, None, None) != 0
+    assert b"mode" in E.lib.pycbf_last_error()
+    assert E.lib.pycbf_make_tabs_2d(0, 1, None, 1, None, None, 1, None, 1, 1540.0, 1.5, None,
+                                    None, None, None, None) != 0
     assert b"NULL" in E.lib.pycbf_last_error()
 
 
