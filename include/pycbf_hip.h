@@ -158,6 +158,16 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
                        double fnum, float *tautx, float *apodtx, float *taurx, float *apodrx,
                        void *stream);
 
+/* FFT-window demodulation of beamformed RF to IQ (fft_tukey_demod of
+ * examples/SyntheticVsTabbed.ipynb, cell 1, along the last axis) as one
+ * real product on the device: iq (rows, ncol) = rf (rows, n) @ op (n, ncol),
+ * float32, FP32 MFMA accumulation.  op is the demodulation operator the host
+ * builds once per (n, band, window): column 2m holds Re and column 2m+1 Im of
+ * output sample m, so iq reads as complex64 (rows, ncol/2).  All pointers
+ * are device pointers, rows row-major and dense; ncol <= 64 * 65535. */
+int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *op, int64_t ncol,
+                      float *iq, void *stream);
+
 /* HOST function.  Spatially compact processing order for np pixel positions
  * (host pvec (np, ndim)) written to host perm (np).  No reference counterpart:
  * the reference launches one thread per sample in pixel-index order. */

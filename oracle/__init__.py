@@ -16,6 +16,10 @@ Contents
   read-only reference tree (git-ignored, rebuilt only where the tree exists).
 * ``numerics``: FP64 NumPy restatements of DMAS / SLSC and the k-order ``S``
   operator.
+* ``diff64``: FP64 torch-autograd restatement of the Tabbed DAS and the
+  straight-ray delays.
+* ``tabs``: FP64 NumPy restatement of ``make_tabs_and_apods_2D``.
+* ``demod``: FP64 NumPy restatement of the notebooks' ``fft_tukey_demod``.
 
 Pinning status of each function is listed in ``oracle/README`` style in the
 module docstrings of the tests that pin them (tests/test_oracle_*.py).

@@ -139,6 +139,11 @@ int launch_tabs_2d(int mode, int ntx, const double *txc, int nrx, const double *
                    double fnum, float *tautx, float *apodtx, float *taurx, float *apodrx,
                    hipStream_t s);
 
+// ------------------------------------------------------------ demodulation ---
+// C (R x N) = A (R x K) @ B (K x N), row-major float32, FP32 MFMA
+int launch_demod_gemm(const float *A, int64_t R, int K, const float *B, int N, float *C,
+                      hipStream_t s);
+
 // ------------------------------------------------------------------ DMAS ---
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                 int nlag, int dmastype, int sumlags, float *out, hipStream_t s);

@@ -1,0 +1,118 @@
+"""FFT-window demodulation of beamformed RF to IQ on the GPU.
+
+``fft_tukey_demod`` takes the arguments of the notebooks' function of the
+same name (examples/SyntheticVsTabbed.ipynb, cell 1): rfft along ``axis``,
+keep the bins from the one nearest ``kmin`` to the one nearest ``kmax``,
+weight them with a Tukey window, ifftshift them to base band (``shiftit``)
+and ifft.  That chain is linear in the samples of each line, so it is one
+fixed operator M (n x L complex) per (n, band, window):
+
+    M[k, m] = 1/L * sum_b w_b exp(-2 pi i k (ikmin + b) / n) exp(2 pi i s(b) m / L)
+
+with s(b) the position of bin b after ifftshift ((b - L//2) mod L, or b
+without the shift).  The host builds M once in float64 from that closed form
+(cached per device), and ``pycbf_demod_apply`` computes iq = rf @ M for all
+lines at once on MFMA, in float32.  The notebook returns complex128 from
+NumPy or CuPy input; here the result is complex64 (a CUDA tensor for tensor
+input, a NumPy array for NumPy input).
+"""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from . import _engine as E
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def band(n: int, kmin: float, kmax: float, ks: float):
+    """(ikmin, ikmax): the kept rfft bins [ikmin, ikmax) of an n-sample axis
+    sampled at ks (the nearest bins to kmin and kmax, both kept)."""
+    freqs = np.fft.rfftfreq(int(n), 1 / ks)
+    ikmin = int(np.argmin(np.abs(freqs - kmin)))
+    ikmax = int(np.argmin(np.abs(freqs - kmax))) + 1
+    return ikmin, ikmax
+
+
+def demod_operator(n: int, kmin: float, kmax: float, ks: float, alpha: float = 0.25,
+                   shiftit: bool = True) -> np.ndarray:
+    """The demodulation as a complex128 (n, L) matrix: iq = rf @ M."""
+    from scipy.signal.windows import tukey
+    ikmin, ikmax = band(n, kmin, kmax, ks)
+    L = ikmax - ikmin
+    if L < 1:
+        raise ValueError(f"empty band: bins [{ikmin}, {ikmax}) for kmin={kmin}, kmax={kmax}")
+    w = tukey(L, alpha=alpha)
+    b = np.arange(L)
+    pos = (b - L // 2) % L if shiftit else b
+    k = np.arange(n)
+    # forward DFT of each sample onto the kept bins (exponent reduced mod n exactly)
+    fwd = np.exp(-2j * np.pi * ((k[:, None] * (ikmin + b)[None, :]) % n) / n) * w[None, :]
+    m = np.arange(L)
+    inv = np.exp(2j * np.pi * ((pos[:, None] * m[None, :]) % L) / L) / L
+    return fwd @ inv
+
+
+@functools.lru_cache(maxsize=16)
+def _device_operator(n, kmin, kmax, ks, alpha, shiftit, dev_index):
+    torch = _torch()
+    M = demod_operator(n, kmin, kmax, ks, alpha, shiftit)
+    op = np.empty((n, 2 * M.shape[1]), np.float32)
+    op[:, 0::2] = M.real
+    op[:, 1::2] = M.imag
+    return torch.as_tensor(op, device=torch.device("cuda", dev_index))
+
+
+def fft_tukey_demod(rfdata, kmin, kmax, ks, alpha=0.25, axis=-1, shiftit=True):
+    """IQ of real ``rfdata`` along ``axis``: complex64, the axis shortened to
+    the L kept bins."""
+    torch = _torch()
+    as_numpy = not isinstance(rfdata, torch.Tensor)
+    if as_numpy:
+        arr = np.asarray(rfdata)
+        if np.iscomplexobj(arr):
+            raise TypeError("fft_tukey_demod needs real RF data")
+        x = torch.as_tensor(np.ascontiguousarray(arr, np.float32), device="cuda")
+    else:
+        if rfdata.is_complex():
+            raise TypeError("fft_tukey_demod needs real RF data")
+        if rfdata.device.type != "cuda":
+            raise ValueError("fft_tukey_demod needs a CUDA tensor or a NumPy array")
+        x = rfdata.to(torch.float32)
+    if x.dim() < 1:
+        raise ValueError("fft_tukey_demod needs at least one axis")
+    dev = x.device
+    with torch.cuda.device(dev):
+        x = torch.movedim(x, axis, -1).contiguous()
+        n = x.shape[-1]
+        op = _device_operator(int(n), float(kmin), float(kmax), float(ks), float(alpha),
+                              bool(shiftit), dev.index if dev.index is not None else 0)
+        ncol = op.shape[1]
+        lead = tuple(x.shape[:-1])
+        rows = int(np.prod(lead)) if lead else 1
+        out = torch.empty(lead + (ncol,), dtype=torch.float32, device=dev)
+        rc = E.lib.pycbf_demod_apply(E.ptr(x), rows, n, E.ptr(op), ncol, E.ptr(out),
+                                     E.stream_of(dev))
+        E.check(rc, "pycbf_demod_apply")
+        iq = torch.view_as_complex(out.view(lead + (ncol // 2, 2)))
+        iq = torch.movedim(iq, -1, axis)
+    if as_numpy:
+        return iq.cpu().numpy()
+    return iq
+
+
+def zout_fft_tukey_demod(zout, kmin, kmax):
+    """Axial coordinates of the samples ``fft_tukey_demod`` returns along a
+    z axis sampled at ``zout`` (same cell of the notebook)."""
+    zout = np.asarray(zout, np.float64)
+    dz = zout[1] - zout[0]
+    ks = np.fft.rfftfreq(len(zout), dz)
+    dk = ks[1] - ks[0]
+    ikmin = np.argmin(np.abs(ks - kmin))
+    ikmax = np.argmin(np.abs(ks - kmax))
+    return zout[0] + np.arange(ikmax - ikmin + 1) / (dk * (ikmax - ikmin))

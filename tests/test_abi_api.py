@@ -106,6 +106,12 @@ def test_host_entry_points_report_argument_errors():
     assert E.lib.pycbf_make_tabs_2d(0, 1, None, 1, None, None, 1, None, 1, 1540.0, 1.5, None,
                                     None, None, None, None) != 0
     assert b"NULL" in E.lib.pycbf_last_error()
+    # so does the demodulation product
+    assert E.lib.pycbf_demod_apply(None, 4, 16, None, 8, None, None) != 0
+    assert b"NULL" in E.lib.pycbf_last_error()
+    assert E.lib.pycbf_demod_apply(None, 4, 0, None, 8, None, None) != 0
+    assert b"n >= 1" in E.lib.pycbf_last_error()
+    assert E.lib.pycbf_demod_apply(None, 0, 16, None, 8, None, None) == 0
 
 
 def test_workspace_query_is_host_only_and_grows_with_frames():

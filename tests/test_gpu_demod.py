@@ -1,0 +1,111 @@
+"""Device demodulation (pycbf_demod_apply through pycbf_amd.gpu.demod)
+against the oracle's FFT chain, and the FP32 MFMA product against float64 on
+ragged shapes."""
+import ctypes
+import time
+
+import numpy as np
+import pytest
+
+from oracle import demod as od
+
+pytestmark = pytest.mark.gpu
+
+U32 = 2.0 ** -24
+
+
+@pytest.mark.parametrize("rows,k,ncol", [(1, 1, 1), (63, 17, 130), (65, 100, 3), (130, 15, 64),
+                                         (200, 513, 66), (64, 64, 64)])
+def test_product_within_fp32_bound(rows, k, ncol):
+    """|iq - rf @ op| <= (k + 2) u (|rf| @ |op|): the bound of any float32
+    summation order with float32 products."""
+    import torch
+    from pycbf_amd.gpu import _engine as E
+    print(f"demod product rows={rows} k={k} ncol={ncol}", flush=True)
+    rng = np.random.default_rng(rows * 1000 + k)
+    a = rng.standard_normal((rows, k)).astype(np.float32)
+    b = rng.standard_normal((k, ncol)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    ad, bd = torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev)
+    out = torch.full((rows, ncol), float("nan"), dtype=torch.float32, device=dev)
+    rc = E.lib.pycbf_demod_apply(E.ptr(ad), rows, k, E.ptr(bd), ncol, E.ptr(out),
+                                 E.stream_of(dev))
+    E.check(rc, "pycbf_demod_apply")
+    got = out.cpu().numpy().astype(np.float64)
+    want = a.astype(np.float64) @ b.astype(np.float64)
+    bound = (k + 2) * U32 * (np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64))
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - want) <= bound + 1e-30)
+
+
+def _lines(nlines, n, dz, seed):
+    """Band-limited RF lines (5-12 MHz echoes at 1540 m/s) plus noise."""
+    rng = np.random.default_rng(seed)
+    z = np.arange(n) * dz
+    k0 = 2 * 7.5e6 / 1540.0
+    x = np.zeros((nlines, n))
+    for i in range(nlines):
+        for zc, amp in zip(rng.uniform(z[0], z[-1], 12), rng.standard_normal(12)):
+            x[i] += amp * np.exp(-((z - zc) / (3 / k0)) ** 2) * np.cos(2 * np.pi * k0 * (z - zc))
+    return x + 0.01 * rng.standard_normal(x.shape)
+
+
+@pytest.mark.parametrize("n,shiftit", [(1024, True), (777, True), (512, False)])
+def test_demod_matches_oracle(n, shiftit):
+    import torch
+    from pycbf_amd.gpu.demod import fft_tukey_demod
+    print(f"demod lines n={n} shiftit={shiftit}", flush=True)
+    dz = 20e-6
+    kmin, kmax, ks = 2 * 5e6 / 1540, 2 * 12e6 / 1540, 1 / dz
+    x = _lines(96, n, dz, n)
+    want = od.fft_tukey_demod(x, kmin, kmax, ks, shiftit=shiftit)
+    got = fft_tukey_demod(torch.as_tensor(x.astype(np.float32), device="cuda"), kmin, kmax, ks,
+                          shiftit=shiftit)
+    assert got.dtype == torch.complex64 and tuple(got.shape) == want.shape
+    g = got.cpu().numpy()
+    peak = np.abs(want).max()
+    assert peak > 0
+    # float32 input rounding plus float32 accumulation over n samples
+    assert np.abs(g - want).max() <= 2e-5 * peak
+
+
+def test_demod_axis_and_numpy_input():
+    """Axis 0 of a 3-D NumPy array (beamformed (nz, nx, frames)): NumPy
+    complex64 out, equal to the oracle along that axis."""
+    from pycbf_amd.gpu.demod import fft_tukey_demod
+    dz = 25e-6
+    kmin, kmax, ks = 2 * 5e6 / 1540, 2 * 12e6 / 1540, 1 / dz
+    x = _lines(24, 600, dz, 9).reshape(4, 6, 600).transpose(2, 1, 0).copy()  # (600, 6, 4)
+    got = fft_tukey_demod(x, kmin, kmax, ks, axis=0)
+    want = od.fft_tukey_demod(x, kmin, kmax, ks, axis=0)
+    assert isinstance(got, np.ndarray) and got.dtype == np.complex64
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+
+def test_demod_no_lines():
+    import torch
+    from pycbf_amd.gpu.demod import fft_tukey_demod
+    x = torch.zeros((0, 256), dtype=torch.float32, device="cuda")
+    iq = fft_tukey_demod(x, 0.1, 0.3, 1.0)
+    ikmin, ikmax = od.band(256, 0.1, 0.3, 1.0)
+    assert tuple(iq.shape) == (0, ikmax - ikmin)
+
+
+def test_demod_full_image_timing():
+    """512 lines x 4096 axial samples (a 512-line image at lambda/8), band
+    5-12 MHz: one product after the operator is cached."""
+    import torch
+    from pycbf_amd.gpu.demod import fft_tukey_demod
+    dz = 1540 / (2 * 40e6) / 2
+    kmin, kmax, ks = 2 * 5e6 / 1540, 2 * 12e6 / 1540, 1 / dz
+    x = torch.randn((512, 4096), dtype=torch.float32, device="cuda")
+    fft_tukey_demod(x, kmin, kmax, ks)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        iq = fft_tukey_demod(x, kmin, kmax, ks)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e2
+    print(f"demod 512 x 4096 -> {tuple(iq.shape)}: {ms:.3f} ms per call", flush=True)
+    assert bool(torch.isfinite(torch.view_as_real(iq)).all())
