@@ -75,10 +75,17 @@ __device__ __forceinline__ float wave_sum(float v)
 // block: lane (j + 32h) holds G[i][j] in register r, i = 8(r/4) + 4h + r%4.
 // Entry r of a lane is taken from G0 when j >= i (tsel >= 8(r/4) + r%4 with
 // tsel = j - 4h) and is fetched by lane 32h + ((j + i) & 31), whose byte
-// address for ds_bpermute is baddr[r].
+// address for ds_bpermute is baddr(r) = ((4(j + 4h) + 4c_r) & 124) | 128h
+// with c_r = 8(r/4) + r%4 (computed at the use: two VALU operations instead
+// of sixteen live registers).
 struct BandLanes {
     int tsel;
-    int baddr[16];
+    int base4;  // 4(j + 4h)
+    int hb;     // 128h
+    __device__ __forceinline__ int baddr(int r) const
+    {
+        return ((base4 + 4 * (8 * (r >> 2) + (r & 3))) & 124) | hb;
+    }
 };
 
 __device__ __forceinline__ BandLanes band_lanes()
@@ -87,11 +94,8 @@ __device__ __forceinline__ BandLanes band_lanes()
     const int j = lane & 31, h = lane >> 5;
     BandLanes b;
     b.tsel = j - 4 * h;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-        b.baddr[r] = (32 * h + ((j + i) & 31)) << 2;
-    }
+    b.base4 = 4 * (j + 4 * h);
+    b.hb = 128 * h;
     return b;
 }
 
@@ -291,7 +295,7 @@ __device__ __forceinline__ void slsc_pixel(const SlscArgs &a, unsigned char *zbu
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const float v = bl.tsel >= 8 * (r >> 2) + (r & 3) ? g0[r] : g1[r];
-                acc += __int_as_float(__builtin_amdgcn_ds_bpermute(bl.baddr[r], __float_as_int(v)));
+                acc += __int_as_float(__builtin_amdgcn_ds_bpermute(bl.baddr(r), __float_as_int(v)));
             }
             acc += __shfl_xor(acc, 32);
             const float got = __shfl(acc, mylag & 31);
@@ -441,23 +445,31 @@ __device__ __forceinline__ float lpp_sum(float v)
 }
 
 // Fast path of the MFMA precisions for windows that tile the axis
-// (dsfz == kwin == KP): a wave takes PZ = 32/KP consecutive output pixels of a
-// column, whose windows are 32 consecutive samples of every channel.  Those
-// are read as channel rows: lane l holds samples 4*(l%8) .. +3 of channel
-// 8q + l/8, so one load instruction covers eight channels x 128 contiguous
-// bytes (lane-per-channel loads of 16 bytes make each lane its own L2
-// request), and a window of pixel p is held by the KP/4 lanes l%8 = p*KP/4 ..
-// of each row.  For K = 8 the two lanes of a window swap halves (DPP) so that
+// (dsfz == kwin == KP): a wave takes PZ = RUN/KP consecutive output pixels of
+// a column, whose windows are RUN (16 or 32) consecutive samples of every
+// channel.  Those are read as channel rows: with S = RUN/4 segments per row,
+// lane l holds samples 4*(l%S) .. +3 of channel (64/S)*q + l/S, so one load
+// instruction covers 64/S channels x 4*RUN contiguous bytes (lane-per-channel
+// loads of 16 bytes make each lane its own L2 request), and a window of pixel
+// p is held by the KP/4 lanes l%S = p*KP/4 .. of each row.  For K = 8 the two lanes of a window swap halves (DPP) so that
 // each lane normalises one whole window of two rows; for K = 16 / 32 the
 // lanes of a window reduce its mean and variance with shuffles.  The
 // normalised samples go to the pixel's compacted buffer, and each pixel's lag
 // sums then run as in k_slsc with every block operand read before the first
 // product.  The wave's selection words are read with scalar loads.
-template <int PREC, int KP, int NG>
-__global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
+template <int PREC, int KP, int NG, int RUN>
+// waves_per_eu(5): a register budget that keeps the MFMA accumulators in
+// VGPRs (with the default budget they go to AGPRs and the kernel runs four
+// waves per SIMD); the 32-sample runs of more than 128 channels keep the
+// default (they would spill)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NG == 2 || RUN <= 16 ? 5 : 1))) void k_slsc_rows(SlscArgs a)
 {
     static_assert(PREC != 0, "the channel-row path feeds the MFMA kernels");
-    constexpr int PZ = 32 / KP, LPP = KP / 4, NQ = NG * 8;
+    static_assert(RUN == 8 || RUN == 16 || RUN == 32, "runs of 8, 16 or 32 samples");
+    static_assert(RUN >= KP, "a run holds whole windows");
+    // S segments of 4 samples per channel row, CPR channels per load
+    constexpr int PZ = RUN / KP, LPP = KP / 4, S = RUN / 4, CPR = 64 / S, NQ = NG * 64 / CPR;
     extern __shared__ __align__(16) unsigned char smem[];
     const int nrx = (int)a.nrx;
     const int lane = threadIdx.x & 63;
@@ -471,7 +483,7 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
     const float *col = a.im + ((int64_t)f * a.nrx * a.nx + ix) * a.nz;  // + r * rstride + z
     const int izo0 = (blockIdx.x * nwave + wave) * PZ;
 
-    const int c8 = lane >> 3, k8 = lane & 7;
+    const int c8 = lane / S, k8 = lane % S;
     const int pp = k8 / LPP;                  // this lane's pixel
     const int izo = izo0 + pp;
     const bool pix = izo < a.nzo;
@@ -511,8 +523,8 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
     float x[NQ][4];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int r = 8 * q + c8;
-        const bool sel = (m[q / 8] >> (r & 63)) & 1ull;
+        const int r = CPR * q + c8;
+        const bool sel = (m[(CPR * q) / 64] >> (r & 63)) & 1ull;
         const float *xr = col + (int64_t)(sel ? r : 0) * rstride + zl;
         F4s v{{0.0f, 0.0f, 0.0f, 0.0f}};
         if (sel) {
@@ -533,8 +545,8 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
     if constexpr (LPP == 2) {
         // Channel rows q0 = 2t and q1 = 2t+1: the two lanes holding the
         // halves of a window swap one half (DPP), so that lane (c8, k8)
-        // holds the whole 8-sample window of channel 8*(2t + k8%2) + c8 for
-        // its pixel k8/2 and normalises it in registers.
+        // holds the whole 8-sample window of channel CPR*(2t + k8%2) + c8
+        // for its pixel k8/2 and normalises it in registers.
         const int hh = k8 & 1;
         bool inr8[8];
 #pragma unroll
@@ -548,8 +560,10 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
                 xw[e] = hh ? rcv : x[2 * t][e];
                 xw[4 + e] = hh ? x[2 * t + 1][e] : rcv;
             }
-            const int r = 8 * (2 * t + hh) + c8;
-            const bool sel = (m[t / 4] >> (r & 63)) & 1ull;
+            const int r = CPR * (2 * t + hh) + c8;
+            // rows 2t, 2t+1 lie in one 64-bit word (2*CPR divides 64)
+            const int wt = (2 * CPR * t) / 64;
+            const bool sel = (m[wt] >> (r & 63)) & 1ull;
             float s8 = 0.0f;
 #pragma unroll
             for (int e = 0; e < 8; ++e) s8 += xw[e];
@@ -564,9 +578,9 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
             const bool valid = v >= vmin;
             const float rs = valid ? __frsqrt_rn(v) : 0.0f;
             if (sel) {
-                int pos = __popcll(m[t / 4] & ((1ull << (r & 63)) - 1ull));
+                int pos = __popcll(m[wt] & ((1ull << (r & 63)) - 1ull));
 #pragma unroll
-                for (int g = 0; g < t / 4; ++g) pos += __popcll(m[g]);
+                for (int g = 0; g < wt; ++g) pos += __popcll(m[g]);
                 vf[pos] = valid ? 1 : 0;
                 unsigned hv[8];
 #pragma unroll
@@ -581,8 +595,9 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
         // to the pixel's compacted buffer
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int r = 8 * q + c8;
-            const bool sel = (m[q / 8] >> (r & 63)) & 1ull;
+            const int r = CPR * q + c8;
+            const int wq = (CPR * q) / 64;
+            const bool sel = (m[wq] >> (r & 63)) & 1ull;
             float s = (x[q][0] + x[q][1]) + (x[q][2] + x[q][3]);
             s = lpp_sum<LPP>(s);
             const float mu = s * rk;
@@ -598,9 +613,9 @@ __global__ __launch_bounds__(256) void k_slsc_rows(SlscArgs a)
             const float rs = valid ? __frsqrt_rn(v) : 0.0f;
             if (sel) {
                 // position among the pixel's usable channels
-                int pos = __popcll(m[q / 8] & ((1ull << (r & 63)) - 1ull));
+                int pos = __popcll(m[wq] & ((1ull << (r & 63)) - 1ull));
 #pragma unroll
-                for (int g = 0; g < q / 8; ++g) pos += __popcll(m[g]);
+                for (int g = 0; g < wq; ++g) pos += __popcll(m[g]);
                 if (k8 % LPP == 0) vf[pos] = valid ? 1 : 0;
                 const int w0 = 4 * (k8 % LPP);
                 unsigned *zr = reinterpret_cast<unsigned *>(zb) + (pos * KP + w0) / 2;
@@ -675,7 +690,7 @@ int launch_slsc_selection(const uint8_t *mask, int64_t nrx, int64_t nx, int64_t 
 // Per-wave LDS (normalised channels, flags, FP32 reduction rows), waves per
 // workgroup and pixels per wave of a launch.  rows: the channel-row fast path
 // (PZ = 32/KP pixels per wave, one buffer per pixel).
-static void slsc_plan(SlscArgs &a, int prec, int kp, bool rows, int &nwave, size_t &lds)
+static void slsc_plan(SlscArgs &a, int prec, int kp, bool rows, int run, int &nwave, size_t &lds)
 {
     const int nrx = (int)a.nrx;
     const int npad = (nrx + 31) / 32 * 32;
@@ -683,10 +698,16 @@ static void slsc_plan(SlscArgs &a, int prec, int kp, bool rows, int &nwave, size
     const size_t redb = prec == 0 ? 8 * 65 * 4 : 0;
     if (rows) {
         // MFMA precisions only
-        a.pz = 32 / kp;
+        a.pz = run / kp;
         a.wbytes = (int)align_up((size_t)a.pz * (a.zbytes + 256), 16);
-        // up to 40 KB per workgroup keeps four workgroups per CU
-        nwave = std::max(1, std::min(4, 40960 / a.wbytes));
+        // up to 40 KB per workgroup; waves per workgroup measured on 128
+        // channels, 256 x 256 output pixels (FP16, us):
+        //   K = 8,  runs of 16: 1 wave 56.8, 2 46.5, 3 41.6, 4 46.1
+        //   K = 16, runs of 16: 1 wave 56.8, 2 41.8, 4 42.2
+        const int cap = std::max(1, std::min(4, 40960 / a.wbytes));
+        nwave = run == 16 ? (kp == 8 ? 3 : 2) : cap;
+        if (const char *e = getenv("PYCBF_SLSC_NWAVE")) nwave = atoi(e);
+        nwave = std::max(1, std::min(cap, nwave));
     } else {
         a.wbytes = (int)align_up((size_t)a.zbytes + 256 + redb, 16);
         // one pixel per wave: more waves in flight beat longer runs per wave
@@ -714,7 +735,21 @@ static int launch_slsc_t(const SlscArgs &a, bool rows, int nwave, size_t lds, in
     ProfScope ps(slsc_prof_name(PREC), s);
     if constexpr (PREC != 0) {
         if (rows) {
-            hipLaunchKernelGGL((k_slsc_rows<PREC, KP, NG>), grid, dim3(64 * nwave), lds, s, a);
+            if constexpr (KP == 8) {
+                if (a.pz * KP == 8) {
+                    hipLaunchKernelGGL((k_slsc_rows<PREC, KP, NG, 8>), grid, dim3(64 * nwave),
+                                       lds, s, a);
+                    return check_launch("slsc");
+                }
+            }
+            if constexpr (KP <= 16) {
+                if (a.pz * KP == 16) {
+                    hipLaunchKernelGGL((k_slsc_rows<PREC, KP, NG, 16>), grid, dim3(64 * nwave),
+                                       lds, s, a);
+                    return check_launch("slsc");
+                }
+            }
+            hipLaunchKernelGGL((k_slsc_rows<PREC, KP, NG, 32>), grid, dim3(64 * nwave), lds, s, a);
             return check_launch("slsc");
         }
     }
@@ -771,9 +806,17 @@ int launch_slsc(const float *im, int64_t nfr, int64_t nrx, int64_t nx, int64_t n
     // selects the lane-per-channel kernel)
     bool rows = precision != PYCBF_SLSC_FP32 && dsfz == kwin && kwin == kp;
     if (const char *e = getenv("PYCBF_SLSC_ROWS")) rows = rows && atoi(e) != 0;
+    // channel-row runs of 16 samples for K <= 16 (K = 8 on 128 channels, 256 x
+    // 256 pixels: runs of 8 / 16 / 32 take 54.6 / 41.6 / 55.7 us), 32 for
+    // K = 32; PYCBF_SLSC_RUN selects 8, 16 or 32 (at least K)
+    int run = kp <= 16 ? 16 : 32;
+    if (const char *e = getenv("PYCBF_SLSC_RUN")) {
+        const int r = atoi(e);
+        run = (r == 8 || r == 16) && r >= kp ? r : 32;
+    }
     int nwave = 0;
     size_t lds = 0;
-    slsc_plan(a, precision, kp, rows, nwave, lds);
+    slsc_plan(a, precision, kp, rows, run, nwave, lds);
     switch (precision) {
     case PYCBF_SLSC_FP32: return launch_slsc_p<0>(a, kp, rows, nwave, lds, nfr, s);
     case PYCBF_SLSC_MFMA_FP16: return launch_slsc_p<1>(a, kp, rows, nwave, lds, nfr, s);
