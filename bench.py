@@ -29,6 +29,7 @@ if ROOT not in sys.path:
 
 METRIC = "Gsamples/s (tx·rx·px) cubic DAS + SLSC px/s on 1 MI355X vs plain GPU & CPU"
 HBM_PEAK_GBPS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 NTX, NRX, NT, NX, NZ = 75, 128, 4096, 512, 512
 
@@ -215,8 +216,64 @@ def slsc_leg(dev, reps, with_cpu):
                                "sample": f"notebook pixel loop (numpy FP64) on {len(cols)} of "
                                          f"{SLSC_NXO} columns ({n} px), {wall:.2f} s"}
         log(f"slsc cpu baseline: {n / wall:.1f} px/s")
+    dres = demod_leg(dev, imrx.view(NRX * nx, nzf), 40e-3 / (nzf - 1), reps, with_cpu)
     del rf, imrx, sel
     torch.cuda.empty_cache()
+    return res, dres
+
+
+def demod_leg(dev, lines, dz, reps, with_cpu):
+    """Demodulation of the coherence leg's per-receive images to IQ along
+    depth (the notebooks' fft_tukey_demod, 5-12 MHz band at 1540 m/s): every
+    line of 2048 axial samples of all receive channels and columns, one FP32
+    MFMA product per call.  Returns the JSON object of the leg."""
+    import torch
+
+    from pycbf_amd.gpu import _engine as E
+    from pycbf_amd.gpu.demod import band, fft_tukey_demod
+    kmin, kmax, ks = 2 * 5e6 / 1540.0, 2 * 12e6 / 1540.0, 1.0 / dz
+    rows, n = lines.shape
+    ikmin, ikmax = band(n, kmin, kmax, ks)
+    ncol = 2 * (ikmax - ikmin)
+    iq = fft_tukey_demod(lines, kmin, kmax, ks)
+    torch.cuda.synchronize()
+    E.lib.pycbf_profile_reset()
+    E.lib.pycbf_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        iq = fft_tukey_demod(lines, kmin, kmax, ks)
+    torch.cuda.synchronize()
+    call_ms = (time.perf_counter() - t0) * 1e3 / reps
+    E.lib.pycbf_profile_enable(0)
+    k_ms, k_n = E.profile_read("demod_gemm")
+    kms = k_ms / max(k_n, 1)
+    flops = 2.0 * rows * n * ncol
+    tf = flops / (kms * 1e-3) / 1e12
+    log(f"demod: {rows} lines x {n} -> {ncol // 2} IQ, kernel {kms * 1e3:.1f} us "
+        f"({tf:.1f} TFLOP/s), call {call_ms:.3f} ms")
+    res = {
+        "workload": f"fft_tukey_demod of {rows} lines x {n} axial samples (dz {dz * 1e6:.2f} um, "
+                    f"5-12 MHz) to {ncol // 2} IQ samples",
+        "lines_per_s": round(rows / (kms * 1e-3), 1),
+        "kernel_us": round(kms * 1e3, 2),
+        "call_ms": round(call_ms, 3),
+        "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tf / FP32_MFMA_PEAK_TFLOPS, 4)},
+    }
+    if with_cpu:
+        from oracle import demod as odemod
+        sub = lines[:2048].double().cpu().numpy()
+        t0 = time.perf_counter()
+        ref = odemod.fft_tukey_demod(sub, kmin, kmax, ks)
+        wall = time.perf_counter() - t0
+        got = iq[:2048].cpu().numpy()
+        res["max_abs_err_rel_peak"] = float(f"{np.abs(got - ref).max() / np.abs(ref).max():.3g}")
+        res["cpu_baseline"] = {"value": round(sub.shape[0] / wall, 1), "unit": "lines/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"numpy FFT chain (FP64) on 2048 of {rows} lines, "
+                                         f"{wall:.3f} s"}
+        log(f"demod cpu baseline: {sub.shape[0] / wall:.1f} lines/s; error "
+            f"{res['max_abs_err_rel_peak']} of peak")
     return res
 
 
@@ -524,11 +581,11 @@ def main():
                "sample": desc}
         log(f"cpu baseline: {rate:.4f} Gsamples/s on {used} threads ({kind})")
 
-    slsc_res = None
+    slsc_res = demod_res = None
     if not args.no_slsc:
         del rf
         torch.cuda.empty_cache()
-        slsc_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
+        slsc_res, demod_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
     adj_res = None if args.no_adjoint else adjoint_leg(dev, args.adjoint_reps)
     sos_res = None if args.no_sos else sos_leg(dev, args.sos_steps)
     batch_res = None if args.no_batch else batch_leg(dev, args.batch_runs, args.batch_speckle)
@@ -559,6 +616,7 @@ def main():
                       "spline_coefs": round(s_ms / max(s_n, 1), 4),
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},
         "slsc": slsc_res,
+        "demod": demod_res,
         "adjoint": adj_res,
         "sos": sos_res,
         "batch": batch_res,

@@ -12,6 +12,7 @@
 // quarter per wave; k advances in slices of 16 staged in LDS (rf 64 x 16,
 // op 16 x 64), zero-filled past the matrix edges.
 #include "common.h"
+#include "prof.h"
 #include "tiles.h"
 
 namespace pycbf {
@@ -99,6 +100,7 @@ int launch_demod_gemm(const float *A, int64_t R, int K, const float *B, int N, f
     if (R <= 0 || N <= 0) return 0;
     // row tiles on x (up to 2^31 - 1), column tiles on y (ncol < 2^22)
     dim3 grid((unsigned)((R + DT - 1) / DT), (unsigned)((N + DT - 1) / DT));
+    ProfScope ps("demod_gemm", s);
     hipLaunchKernelGGL(k_demod_gemm, grid, dim3(256), 0, s, A, R, K, B, N, C);
     return check_launch("demod gemm");
 }
