@@ -160,12 +160,14 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
 
 /* FFT-window demodulation of beamformed RF to IQ (fft_tukey_demod of
  * examples/SyntheticVsTabbed.ipynb, cell 1, along the last axis) as one
- * real product on the device: iq (rows, ncol) = rf (rows, n) @ op (n, ncol),
- * float32, FP32 MFMA accumulation.  op is the demodulation operator the host
- * builds once per (n, band, window): column 2m holds Re and column 2m+1 Im of
+ * real product on the device: iq[r][c] = sum_k rf[r][k] * opT[c][k], with rf
+ * (rows, n), opT (ncol, n) and iq (rows, ncol), float32; float32-accurate
+ * (three-way BF16 split of both operands on the BF16 matrix units, float32
+ * accumulation).  opT is the transposed demodulation operator the host
+ * builds once per (n, band, window): row 2m holds Re and row 2m+1 Im of
  * output sample m, so iq reads as complex64 (rows, ncol/2).  All pointers
- * are device pointers, rows row-major and dense; ncol <= 64 * 65535. */
-int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *op, int64_t ncol,
+ * are device pointers, rows row-major and dense; ncol <= 128 * 65535. */
+int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *opT, int64_t ncol,
                       float *iq, void *stream);
 
 /* HOST function.  Spatially compact processing order for np pixel positions

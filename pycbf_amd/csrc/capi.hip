@@ -341,15 +341,15 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
                           taurx, apodrx, (hipStream_t)stream);
 }
 
-int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *op, int64_t ncol,
+int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *opT, int64_t ncol,
                       float *iq, void *stream)
 {
     if (rows < 0 || n < 1 || ncol < 1) return fail(PYCBF_EINVAL, "need rows >= 0, n >= 1, ncol >= 1");
     if (n > 0x7fffffffLL) return fail(PYCBF_EINVAL, "n must fit in int32");
-    if (ncol > 64LL * 65535) return fail(PYCBF_EINVAL, "ncol must be <= %d", 64 * 65535);
-    if (rows > 0 && (!rf || !op || !iq)) return fail(PYCBF_EINVAL, "NULL device pointer");
-    if ((rows + 63) / 64 > 0x7fffffffLL) return fail(PYCBF_EINVAL, "too many rows");
-    return launch_demod_gemm(rf, rows, (int)n, op, (int)ncol, iq, (hipStream_t)stream);
+    if (ncol > 128LL * 65535) return fail(PYCBF_EINVAL, "ncol must be <= %d", 128 * 65535);
+    if (rows > 0 && (!rf || !opT || !iq)) return fail(PYCBF_EINVAL, "NULL device pointer");
+    if ((rows + 127) / 128 > 0x7fffffffLL) return fail(PYCBF_EINVAL, "too many rows");
+    return launch_demod_gemm(rf, rows, (int)n, opT, (int)ncol, iq, (hipStream_t)stream);
 }
 
 int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,

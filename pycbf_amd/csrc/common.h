@@ -140,8 +140,9 @@ int launch_tabs_2d(int mode, int ntx, const double *txc, int nrx, const double *
                    hipStream_t s);
 
 // ------------------------------------------------------------ demodulation ---
-// C (R x N) = A (R x K) @ B (K x N), row-major float32, FP32 MFMA
-int launch_demod_gemm(const float *A, int64_t R, int K, const float *B, int N, float *C,
+// C (R x N) = A (R x K) @ Bt (N x K)^T, row-major float32, float32-accurate
+// (three-way BF16 split on the BF16 matrix units)
+int launch_demod_gemm(const float *A, int64_t R, int K, const float *Bt, int N, float *C,
                       hipStream_t s);
 
 // ------------------------------------------------------------------ DMAS ---

@@ -1,16 +1,28 @@
-// FFT-window demodulation of beamformed RF to IQ as one real GEMM on MFMA.
+// FFT-window demodulation of beamformed RF to IQ as one real matrix product.
 //
 // The notebooks' fft_tukey_demod (examples/SyntheticVsTabbed.ipynb, cell 1:
 // rfft, keep a band of bins, Tukey window, ifftshift, ifft) is a fixed linear
-// map from the n samples of each row to L complex samples.  The host builds
-// that map once in float64 as the real matrix op (n, 2L), columns (Re, Im)
-// interleaved; the device computes iq (rows, 2L) = rf (rows, n) @ op with
-// v_mfma_f32_32x32x2_f32 (FP32 inputs, FP32 accumulation), so iq reads as
-// complex64 (rows, L).
+// map from the n samples of each line to L complex samples.  The host builds
+// that map once in float64 and hands it over transposed, as opT (2L, n):
+// row 2m holds Re and row 2m+1 Im of output sample m.  The device computes
+//   iq[r][c] = sum_k rf[r][k] * opT[c][k]
+// so that iq (rows, 2L) reads as complex64 (rows, L).
 //
-// Tiling: a 256-thread workgroup computes a 64 x 64 tile of iq, one 32 x 32
-// quarter per wave; k advances in slices of 16 staged in LDS (rf 64 x 16,
-// op 16 x 64), zero-filled past the matrix edges.
+// Precision: every float32 operand x is split exactly into three BF16 parts,
+// x = x0 + x1 + x2 (x0 = RN(x), x1 = RN(x - x0), x2 = x - x0 - x1, each
+// remainder exact in float32 and the last one exact in BF16), and the
+// product keeps the six cross terms of order >= 2^-16,
+//   a0 b0 + a0 b1 + a1 b0 + a0 b2 + a1 b1 + a2 b0,
+// each an exact product accumulated in float32 by v_mfma_f32_32x32x16_bf16.
+// The dropped terms are below 2^-23 |a||b| per product, so the result is a
+// float32-accurate product on the BF16 matrix units (2.5 PFLOP/s dense)
+// instead of the FP32 ones (157.3 TFLOP/s).
+//
+// Tiling: a 256-thread workgroup computes a 128 x 128 tile of iq; each wave
+// a 64 x 64 quarter as 2 x 2 blocks of 32 x 32.  k advances in slices of 32:
+// the workgroup loads the next slice's rf and opT tiles into registers while
+// the current one (three BF16 planes each, in LDS) feeds the MFMAs; edges are
+// zero-filled.
 #include "common.h"
 #include "prof.h"
 #include "tiles.h"
@@ -19,89 +31,146 @@ namespace pycbf {
 
 namespace {
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
-constexpr int DT = 64;   // workgroup tile (rows and columns)
-constexpr int DK = 16;   // k slice
-}  // namespace
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+constexpr int DM = 128;     // workgroup tile rows (lines) and columns
+constexpr int DK = 32;      // k slice
+constexpr int DP = DK + 8;  // LDS row pitch in BF16 (80 bytes: rows stay 16-byte aligned)
 
-__global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A, int64_t R, int K,
-                                                    const float *__restrict__ B, int N,
-                                                    float *__restrict__ C)
+// rows [row0, row0 + 128) x k [k0, k0 + 32) of a (nrow x K) float32 matrix:
+// thread t holds rows t/8 + 32q (q < 4), k 4(t%8) .. +3
+__device__ __forceinline__ void load_tile(const float *__restrict__ M, int64_t nrow, int K,
+                                          int64_t row0, int k0, F4u (&v)[4])
 {
-    __shared__ float As[DT][DK + 1];
-    __shared__ float Bs[DK][DT + 4];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t row0 = (int64_t)blockIdx.x * DT;
-    const int col0 = blockIdx.y * DT;
-    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
-    f32x16_t acc = {};
-    // loader roles: rf row tid/4, k 4*(tid%4)..+3; op k tid/16, columns 4*(tid%16)..+3
-    const int ar = tid >> 2, ak = 4 * (tid & 3);
-    const int bk = tid >> 4, bc = 4 * (tid & 15);
-    for (int k0 = 0; k0 < K; k0 += DK) {
-        {
-            const int64_t r = row0 + ar;
-            const int k = k0 + ak;
-            F4u v{{0.0f, 0.0f, 0.0f, 0.0f}};
-            if (r < R) {
-                const float *src = A + r * (int64_t)K + k;
-                if (k + 4 <= K) {
-                    v = *reinterpret_cast<const F4u *>(src);
-                } else {
+    const int t = threadIdx.x;
+    const int k = k0 + 4 * (t & 7);
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (k + e < K) v.v[e] = src[e];
-                }
+    for (int q = 0; q < 4; ++q) {
+        const int64_t r = row0 + (t >> 3) + 32 * q;
+        v[q] = F4u{{0.0f, 0.0f, 0.0f, 0.0f}};
+        if (r < nrow) {
+            const float *src = M + r * (int64_t)K + k;
+            if (k + 4 <= K) {
+                v[q] = *reinterpret_cast<const F4u *>(src);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (k + e < K) v[q].v[e] = src[e];
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) As[ar][ak + e] = v.v[e];
         }
-        {
-            const int k = k0 + bk;
-            const int c = col0 + bc;
-            F4u v{{0.0f, 0.0f, 0.0f, 0.0f}};
-            if (k < K) {
-                const float *src = B + (int64_t)k * N + c;
-                if (c + 4 <= N) {
-                    v = *reinterpret_cast<const F4u *>(src);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (c + e < N) v.v[e] = src[e];
-                }
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) Bs[bk][bc + e] = v.v[e];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < DK; kk += 2) {
-            // lane l: A[l%32][kk + l/32] and B[kk + l/32][l%32]
-            const float a = As[wr + (lane & 31)][kk + (lane >> 5)];
-            const float b = Bs[kk + (lane >> 5)][wc + (lane & 31)];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // D[i][j] of the 32 x 32 quarter sits in lane j + 32h, register r, with
-    // i = 8(r/4) + 4h + r%4
-    const int j = lane & 31, h = lane >> 5;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = 8 * (r >> 2) + 4 * h + (r & 3);
-        const int64_t row = row0 + wr + i;
-        const int col = col0 + wc + j;
-        if (row < R && col < N) C[row * (int64_t)N + col] = acc[r];
     }
 }
 
-int launch_demod_gemm(const float *A, int64_t R, int K, const float *B, int N, float *C,
+// three-way BF16 split of the thread's tile part into planes s[0..2]
+__device__ __forceinline__ void store_split(__bf16 *__restrict__ s, const F4u (&v)[4])
+{
+    const int t = threadIdx.x;
+    const int kc = 4 * (t & 7);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bf16x4_t p0, p1, p2;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float x = v[q].v[e];
+            const __bf16 h0 = (__bf16)x;
+            const float r1 = x - (float)h0;
+            const __bf16 h1 = (__bf16)r1;
+            const float r2 = r1 - (float)h1;
+            p0[e] = h0;
+            p1[e] = h1;
+            p2[e] = (__bf16)r2;
+        }
+        const int o = ((t >> 3) + 32 * q) * DP + kc;
+        *reinterpret_cast<bf16x4_t *>(s + o) = p0;
+        *reinterpret_cast<bf16x4_t *>(s + DM * DP + o) = p1;
+        *reinterpret_cast<bf16x4_t *>(s + 2 * DM * DP + o) = p2;
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A, int64_t R, int K,
+                                                    const float *__restrict__ Bt, int N,
+                                                    float *__restrict__ C)
+{
+    __shared__ __align__(16) __bf16 As[3 * DM * DP];
+    __shared__ __align__(16) __bf16 Bs[3 * DM * DP];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t row0 = (int64_t)blockIdx.x * DM;
+    const int col0 = blockIdx.y * DM;
+    const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+    const int j = lane & 31, h = lane >> 5;
+
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x16_t{};
+
+    F4u va[4], vb[4];
+    load_tile(A, R, K, row0, 0, va);
+    load_tile(Bt, N, K, col0, 0, vb);
+    for (int k0 = 0; k0 < K; k0 += DK) {
+        store_split(As, va);
+        store_split(Bs, vb);
+        __syncthreads();
+        if (k0 + DK < K) {
+            // next slice in flight while this one multiplies
+            load_tile(A, R, K, row0, k0 + DK, va);
+            load_tile(Bt, N, K, col0, k0 + DK, vb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < DK; kk += 16) {
+            // lane l: row (or column) l%32 of the block, k kk + 8(l/32) .. +7
+            bf16x8_t a[2][3], b[2][3];
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    const int oa = p * DM * DP + (wr + 32 * bi + j) * DP + kk + 8 * h;
+                    const int ob = p * DM * DP + (wc + 32 * bi + j) * DP + kk + 8 * h;
+                    a[bi][p] = *reinterpret_cast<const bf16x8_t *>(As + oa);
+                    b[bi][p] = *reinterpret_cast<const bf16x8_t *>(Bs + ob);
+                }
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    // small terms first
+                    f32x16_t c = acc[bi][bj];
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][2], b[bj][0], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][1], b[bj][1], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][0], b[bj][2], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][1], b[bj][0], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][0], b[bj][1], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[bi][0], b[bj][0], c, 0, 0, 0);
+                    acc[bi][bj] = c;
+                }
+        }
+        __syncthreads();
+    }
+    // D[i][j] of a 32 x 32 block sits in lane j + 32h, register r, with
+    // i = 8(r/4) + 4h + r%4
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+                const int64_t row = row0 + wr + 32 * bi + i;
+                const int col = col0 + wc + 32 * bj + j;
+                if (row < R && col < N) C[row * (int64_t)N + col] = acc[bi][bj][r];
+            }
+}
+
+int launch_demod_gemm(const float *A, int64_t R, int K, const float *Bt, int N, float *C,
                       hipStream_t s)
 {
     if (R <= 0 || N <= 0) return 0;
-    // row tiles on x (up to 2^31 - 1), column tiles on y (ncol < 2^22)
-    dim3 grid((unsigned)((R + DT - 1) / DT), (unsigned)((N + DT - 1) / DT));
+    // row tiles on x (up to 2^31 - 1), column tiles on y
+    dim3 grid((unsigned)((R + DM - 1) / DM), (unsigned)((N + DM - 1) / DM));
     ProfScope ps("demod_gemm", s);
-    hipLaunchKernelGGL(k_demod_gemm, grid, dim3(256), 0, s, A, R, K, B, N, C);
+    hipLaunchKernelGGL(k_demod_gemm, grid, dim3(256), 0, s, A, R, K, Bt, N, C);
     return check_launch("demod gemm");
 }
 

@@ -12,7 +12,8 @@ fixed operator M (n x L complex) per (n, band, window):
 with s(b) the position of bin b after ifftshift ((b - L//2) mod L, or b
 without the shift).  The host builds M once in float64 from that closed form
 (cached per device), and ``pycbf_demod_apply`` computes iq = rf @ M for all
-lines at once on MFMA, in float32.  The notebook returns complex128 from
+lines at once, float32-accurate on the BF16 matrix units (a three-way BF16
+split of both operands).  The notebook returns complex128 from
 NumPy or CuPy input; here the result is complex64 (a CUDA tensor for tensor
 input, a NumPy array for NumPy input).
 """
@@ -60,12 +61,14 @@ def demod_operator(n: int, kmin: float, kmax: float, ks: float, alpha: float = 0
 
 @functools.lru_cache(maxsize=16)
 def _device_operator(n, kmin, kmax, ks, alpha, shiftit, dev_index):
+    """The operator as the device takes it: transposed, (2L, n) float32 with
+    row 2m = Re and row 2m+1 = Im of output sample m."""
     torch = _torch()
     M = demod_operator(n, kmin, kmax, ks, alpha, shiftit)
-    op = np.empty((n, 2 * M.shape[1]), np.float32)
-    op[:, 0::2] = M.real
-    op[:, 1::2] = M.imag
-    return torch.as_tensor(op, device=torch.device("cuda", dev_index))
+    opT = np.empty((2 * M.shape[1], n), np.float32)
+    opT[0::2] = M.real.T
+    opT[1::2] = M.imag.T
+    return torch.as_tensor(opT, device=torch.device("cuda", dev_index))
 
 
 def fft_tukey_demod(rfdata, kmin, kmax, ks, alpha=0.25, axis=-1, shiftit=True):
@@ -92,7 +95,7 @@ def fft_tukey_demod(rfdata, kmin, kmax, ks, alpha=0.25, axis=-1, shiftit=True):
         n = x.shape[-1]
         op = _device_operator(int(n), float(kmin), float(kmax), float(ks), float(alpha),
                               bool(shiftit), dev.index if dev.index is not None else 0)
-        ncol = op.shape[1]
+        ncol = op.shape[0]
         lead = tuple(x.shape[:-1])
         rows = int(np.prod(lead)) if lead else 1
         out = torch.empty(lead + (ncol,), dtype=torch.float32, device=dev)

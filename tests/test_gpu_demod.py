@@ -1,6 +1,6 @@
 """Device demodulation (pycbf_demod_apply through pycbf_amd.gpu.demod)
-against the oracle's FFT chain, and the FP32 MFMA product against float64 on
-ragged shapes."""
+against the oracle's FFT chain, and the split-BF16 product against float64
+on ragged shapes."""
 import ctypes
 import time
 
@@ -15,27 +15,33 @@ U32 = 2.0 ** -24
 
 
 @pytest.mark.parametrize("rows,k,ncol", [(1, 1, 1), (63, 17, 130), (65, 100, 3), (130, 15, 64),
-                                         (200, 513, 66), (64, 64, 64)])
+                                         (200, 513, 66), (128, 128, 128), (257, 33, 129)])
 def test_product_within_fp32_bound(rows, k, ncol):
-    """|iq - rf @ op| <= (k + 2) u (|rf| @ |op|): the bound of any float32
-    summation order with float32 products."""
+    """iq = rf @ opT^T from the three-way BF16 split: six exact partial
+    products per term (6k summands in float32) plus the dropped cross terms
+    (below 2^-23 |a||b| each) stay within (6k + 4) u (|rf| @ |opT|^T); a
+    transposed operand or a wrong lane layout misses it by orders of
+    magnitude.  Operands span 2^-20 .. 2^20 so the split sees every binade."""
     import torch
     from pycbf_amd.gpu import _engine as E
     print(f"demod product rows={rows} k={k} ncol={ncol}", flush=True)
     rng = np.random.default_rng(rows * 1000 + k)
-    a = rng.standard_normal((rows, k)).astype(np.float32)
-    b = rng.standard_normal((k, ncol)).astype(np.float32)
+    a = (rng.standard_normal((rows, k)) * 2.0 ** rng.integers(-20, 21, (rows, k))).astype(np.float32)
+    bt = (rng.standard_normal((ncol, k)) * 2.0 ** rng.integers(-20, 21, (ncol, k))).astype(np.float32)
     dev = torch.device("cuda", 0)
-    ad, bd = torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev)
+    ad, bd = torch.as_tensor(a, device=dev), torch.as_tensor(bt, device=dev)
     out = torch.full((rows, ncol), float("nan"), dtype=torch.float32, device=dev)
     rc = E.lib.pycbf_demod_apply(E.ptr(ad), rows, k, E.ptr(bd), ncol, E.ptr(out),
                                  E.stream_of(dev))
     E.check(rc, "pycbf_demod_apply")
     got = out.cpu().numpy().astype(np.float64)
-    want = a.astype(np.float64) @ b.astype(np.float64)
-    bound = (k + 2) * U32 * (np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64))
+    a64, b64 = a.astype(np.float64), bt.astype(np.float64)
+    want = a64 @ b64.T
+    bound = (6 * k + 4) * U32 * (np.abs(a64) @ np.abs(b64).T)
     assert np.all(np.isfinite(got))
     assert np.all(np.abs(got - want) <= bound + 1e-30)
+    # typical error is that of a float32 product, far inside the bound
+    assert np.median(np.abs(got - want) / (np.abs(a64) @ np.abs(b64).T + 1e-300)) < 4 * U32 * np.sqrt(k)
 
 
 def _lines(nlines, n, dz, seed):
