@@ -466,6 +466,18 @@ def batch_leg(dev, runs, speckle):
     lat = batch.latencies_ms()
     log(f"batch: {ms:.2f} ms/run, {batch.samples / (ms * 1e-3) / 1e9:.1f} Gsamples/s, latency p50 "
         f"{np.percentile(lat, 50):.2f} p99 {np.percentile(lat, 99):.2f} ms")
+    # algorithmic bytes of the gather, per request: 16 B per cubic sample, the
+    # RF once, the pixel positions, and the output (summed image, or the
+    # per-receive images of an SLSC request)
+    algo = 0.0
+    for (steers, xele, ntx, nrx, side), kind in zip(specs, kinds):
+        npx = side * side
+        algo += (16.0 * ntx * nrx * npx + 4.0 * ntx * nrx * NT + 8.0 * npx
+                 + 4.0 * npx * (nrx if kind == "slsc" else 1))
+    g_ms = grid[(tile, rchunk)]
+    g_achieved = algo / (g_ms * 1e-3) / 1e9
+    log(f"batch gather roofline: {algo / 1e9:.1f} GB algorithmic, {g_achieved:.0f} GB/s "
+        f"({g_achieved / HBM_PEAK_GBPS:.3f} of HBM)")
     res = {
         "workload": "ragged batch of 64 seeded requests (48 cubic DAS, 16 SLSC lags 1..20), grids "
                     "128..512 square, Ntx 1..75, Nrx 64..256, Nt 4096; one persistent gather "
@@ -476,6 +488,9 @@ def batch_leg(dev, runs, speckle):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)},
+        "roofline": {"bound": "hbm", "achieved": round(g_achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(g_achieved / HBM_PEAK_GBPS, 4),
+                     "kernel": "batch_gather", "kernel_ms": round(g_ms, 3)},
         "autotune": {"tile": tile, "rchunk": rchunk,
                      "gather_ms": {f"{t}x{t}/{c}": round(v, 3) for (t, c), v in grid.items()}},
         "data": f"seeded channel data, 50 points + {speckle} speckle scatterers per request",
