@@ -206,7 +206,8 @@ int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int
                int nlag, int dmastype, int sumlags, float *out, void *stream);
 
 /* Bytes of workspace pycbf_das_tabbed_bwd needs (cubic: spline table and
- * cotangent bins of every channel; all kinds: per-tile delay ranges). */
+ * cotangent bins of every channel; korder_cubic: Hermite table and the
+ * cotangents of its four coefficients; all kinds: per-tile delay ranges). */
 size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
                                      int interp_kind);
 
@@ -216,16 +217,18 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
  * a loss with respect to out (layout by sumflag as pycbf_das_tabbed), writes
  *   grf      (nfr, ntx, nrx, nt)  d loss / d rf
  *   gtautx, gapodtx (ntx, np); gtaurx, gapodrx (nrx, np)
- * Any output pointer may be NULL (not computed).  interp_kind nearest (usf 1;
- * its delay gradient is 0), linear or cubic (natural spline: the adjoint of
- * the spline prefilter is applied to grf).  Samples with |apodtx*apodrx| <
- * thresh contribute nothing, as in the forward. */
+ * Any output pointer may be NULL (not computed).  interp_kind nearest
+ * (param = usf = 1; its delay gradient is 0), linear, cubic (natural spline:
+ * the adjoint of the spline prefilter is applied to grf) or korder_cubic
+ * (param = k, S the (k, k) operator as for pycbf_das_tabbed; the adjoint of
+ * the Hermite coefficient construction is applied to grf).  Samples with
+ * |apodtx*apodrx| < thresh contribute nothing, as in the forward. */
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          const float *tautx, const float *apodtx, const float *taurx,
-                         const float *apodrx, int64_t np, int interp_kind, float thresh,
-                         int sumflag, const float *gout, float *grf, float *gtautx,
-                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
-                         size_t workspace_bytes, void *stream);
+                         const float *apodrx, int64_t np, int interp_kind, int param,
+                         const float *S, float thresh, int sumflag, const float *gout,
+                         float *grf, float *gtautx, float *gapodtx, float *gtaurx,
+                         float *gapodrx, void *workspace, size_t workspace_bytes, void *stream);
 
 /* Straight-ray delays through a slowness map (no reference counterpart: the
  * reference's examples/rough_sos_correction.ipynb uses an external eikonal

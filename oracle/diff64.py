@@ -13,6 +13,10 @@ delays and apodisations:
   a = (t0 + dt (j+1) - x) / dt,  b = (x - (t0 + dt j)) / dt
   linear: v = a y_j + b y_j+1;  cubic: v += dt^2/6 ((a^3-a) y2_j + (b^3-b) y2_j+1)
   nearest: v = y_k, k = (int)((x - t0)/dt + 0.5), zero unless -1 < (x - t0)/dt + 0.5 < n
+  korder_cubic: t = ((x - t0) - dt j) / dt, v = c0 + c1 t + c2 t^2 + c3 t^3 with the
+    Hermite coefficients of pycbf/gpu/__engines__.cu:292-355 (end values y_j, y_j+1 and
+    derivatives d0, d1 = rows ik0, ik0+1 of S applied to the k-sample window at iy0,
+    placed as :314-328); interval n-1 is the constant y_{n-1}
 with y_n = y2_n = 0 (the zero entry past the trace) and y2 the natural
 spline's second derivatives (y2_0 = y2_{n-1} = 0), solved exactly in float64.
 
@@ -41,12 +45,37 @@ def natural_y2(y: torch.Tensor, dt: float) -> torch.Tensor:
     return torch.cat([z, sol, z], dim=-1)
 
 
+def korder_coefs(rf: torch.Tensor, S) -> torch.Tensor:
+    """Hermite coefficients (..., n, 4) of the k-order cubic through rf (..., n)
+    (pycbf/gpu/__engines__.cu:292-355): for interval j < n-1, c = (y_j, d0,
+    3 (y_j+1 - y_j) - 2 d0 - d1, 2 (y_j - y_j+1) + d0 + d1); c = (y_{n-1}, 0,
+    0, 0) for the last."""
+    import numpy as np
+    S = torch.as_tensor(S, dtype=rf.dtype)
+    k = S.shape[0]
+    n = rf.shape[-1]
+    js = np.arange(n - 1)
+    interior = (k - 1) // 2
+    ik0 = np.where(js < k // 2, js, np.where(js > n - k // 2 - 1, k - n + js, interior))
+    iy0 = np.where(js < k // 2, 0, np.where(js > n - k // 2 - 1, n - k, js - interior))
+    idx = torch.as_tensor(iy0[:, None] + np.arange(k)[None, :])
+    win = rf[..., idx]                                         # (..., n-1, k)
+    d0 = (win * S[torch.as_tensor(ik0)]).sum(-1)
+    d1 = (win * S[torch.as_tensor(ik0 + 1)]).sum(-1)
+    y0, y1 = rf[..., :-1], rf[..., 1:]
+    c = torch.stack([y0, d0, 3.0 * (y1 - y0) - 2.0 * d0 - d1, 2.0 * (y0 - y1) + d0 + d1], -1)
+    z = torch.zeros_like(rf[..., -1:])
+    last = torch.stack([rf[..., -1:], z, z, z], -1)
+    return torch.cat([c, last], dim=-2)
+
+
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="tx_and_rx",
-               thresh=0.0, x32=True):
+               thresh=0.0, x32=True, S=None):
     """rf (nfr, ntx, nrx, n); tables (ntx, np) / (nrx, np); returns float64
     with the sumtype layout of the GPU beamformers (frames first).  x32 sums
     the delays in float32 as the kernels do (same sample indices); x32=False
-    keeps the whole function in float64 (finite-difference checks)."""
+    keeps the whole function in float64 (finite-difference checks).  S: the
+    (k, k) derivative operator of korder_cubic."""
     rf = rf.to(torch.float64)
     nfr, ntx, nrx, n = rf.shape
     if x32:
@@ -83,6 +112,13 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="
         d = qd + 0.5
         inside = ((d > -1.0) & (d < n)).to(torch.float64)
         v = take(y, torch.trunc(d).clamp(0, n - 1).long())
+    elif kind == "korder_cubic":
+        inside = (lo_ok & hi_ok).to(torch.float64)
+        j = torch.floor(qd).clamp(0, n - 1).long()
+        t = ((x - t0) - dt * j.to(torch.float64)) / dt
+        C = korder_coefs(rf, S)                                # (nfr, ntx, nrx, n, 4)
+        cj = torch.gather(C, 3, j[None, ..., None].expand(nfr, *j.shape, 4))
+        v = cj[..., 0] + t * (cj[..., 1] + t * (cj[..., 2] + t * cj[..., 3]))
     else:
         inside = (lo_ok & hi_ok).to(torch.float64)
         j = torch.floor(qd).clamp(0, n - 1).long()

@@ -515,7 +515,7 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 
 // ------------------------------------------------------------- backward ---
 struct BwdPlan {
-    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, mbits = 256, total = 0;
+    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, gc2 = 0, gc3 = 0, mbits = 256, total = 0;
 };
 
 static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind)
@@ -528,8 +528,11 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         b.coef = align_up((size_t)nch * (size_t)(nt + 1) * 8, 256);
         b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
         b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
+    } else if (kind == PYCBF_INTERP_KORDER_CUBIC) {
+        b.coef = align_up((size_t)nch * (size_t)nt * 16, 256);
+        b.gy = b.ge = b.gc2 = b.gc3 = align_up((size_t)nch * (size_t)nt * 4, 256);
     }
-    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.mbits + 6 * 256;
+    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.gc2 + b.gc3 + b.mbits + 8 * 256;
     return b;
 }
 
@@ -542,16 +545,21 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
 
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          const float *tautx, const float *apodtx, const float *taurx,
-                         const float *apodrx, int64_t np, int interp_kind, float thresh,
-                         int sumflag, const float *gout, float *grf, float *gtautx,
-                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
-                         size_t workspace_bytes, void *stream)
+                         const float *apodrx, int64_t np, int interp_kind, int param,
+                         const float *S, float thresh, int sumflag, const float *gout,
+                         float *grf, float *gtautx, float *gapodtx, float *gtaurx,
+                         float *gapodrx, void *workspace, size_t workspace_bytes, void *stream)
 {
     if (interp_kind != PYCBF_INTERP_NEAREST && interp_kind != PYCBF_INTERP_LINEAR &&
-        interp_kind != PYCBF_INTERP_CUBIC)
-        return fail(PYCBF_EINVAL, "the adjoint supports nearest, linear and cubic (got %d)",
+        interp_kind != PYCBF_INTERP_CUBIC && interp_kind != PYCBF_INTERP_KORDER_CUBIC)
+        return fail(PYCBF_EINVAL,
+                    "the adjoint supports nearest, linear, cubic and korder_cubic (got %d)",
                     interp_kind);
-    int rc = validate_common(info, rf, nfr, np, interp_kind, 1, nullptr, sumflag, gout);
+    if (interp_kind == PYCBF_INTERP_NEAREST && param != 1)
+        return fail(PYCBF_EINVAL, "the nearest adjoint needs usf = 1");
+    const int kparam = interp_kind == PYCBF_INTERP_KORDER_CUBIC ? param : 1;
+    int rc = validate_common(info, rf, nfr, np, interp_kind, kparam,
+                             interp_kind == PYCBF_INTERP_KORDER_CUBIC ? S : nullptr, sumflag, gout);
     if (rc) return rc;
     if (!tautx || !apodtx || !taurx || !apodrx)
         return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
@@ -595,6 +603,17 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         b.gstride = nt + 1;
         if ((rc = launch_spline_coefs(rf, nch, nt, dt, u, coef, s))) return rc;
         b.src = (const float *)coef;
+    } else if (interp_kind == PYCBF_INTERP_KORDER_CUBIC) {
+        float4 *coef = (float4 *)cv.take(plan.coef);
+        b.gy = (float *)cv.take(plan.gy);
+        b.ge = (float *)cv.take(plan.ge);
+        b.gc2 = (float *)cv.take(plan.gc2);
+        b.gc3 = (float *)cv.take(plan.gc3);
+        b.gstride = nt;
+        b.S = S;
+        b.k = param;
+        if ((rc = launch_korder_coefs(rf, nch, nt, S, param, coef, s))) return rc;
+        b.src = (const float *)coef;
     } else {
         b.gy = grf;
         b.gstride = nt;
@@ -612,8 +631,10 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         a.geo = 0;
         if ((rc = launch_tile_bounds(a, bounds, s))) return rc;
     }
-    const int gk = interp_kind == PYCBF_INTERP_NEAREST ? GK_NEAREST
-                   : (interp_kind == PYCBF_INTERP_LINEAR ? GK_LINEAR : GK_CUBIC);
+    const int gk = interp_kind == PYCBF_INTERP_NEAREST  ? GK_NEAREST
+                   : interp_kind == PYCBF_INTERP_LINEAR ? GK_LINEAR
+                   : interp_kind == PYCBF_INTERP_CUBIC  ? GK_CUBIC
+                                                        : GK_KORDER;
     return launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
 }
 

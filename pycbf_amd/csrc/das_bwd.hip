@@ -26,7 +26,11 @@
 // e = dt^2/6 y2, and k_spline_adjoint(_seg) adds the transposed spline solve
 //   dy += D2(T^-1 de),  T = tridiag(1, 4, 1),  D2 the second difference,
 // which is the adjoint of e = T^-1 D2 y (the natural spline with y2 = 0 at
-// both ends).
+// both ends).  For korder_cubic the bins hold the cotangents of the four
+// Hermite coefficients of each interval (the value is c.x + c.y t + c.z t^2
+// + c.w t^3), and k_korder_adjoint transposes their construction from the
+// samples (k_korder_coefs: end values and S-derivatives over a k-sample
+// window).
 #include <math.h>
 
 #include <algorithm>
@@ -53,6 +57,14 @@ __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t
         if (!nearest_index(x, t0, dt, n, it)) return false;
         v = src[row * n + it];
         d = 0.0f;
+        return true;
+    } else if (GK == GK_KORDER) {
+        int j = 0;
+        if (!knot_index(x, t0, dt, tmax, n, j)) return false;
+        const float t = ((x - t0) - dt * (float)j) * rdt;
+        const float4 c = reinterpret_cast<const float4 *>(src)[row * n + j];
+        v = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, t, c.z), t, c.y), t, c.x);
+        d = __builtin_fmaf(__builtin_fmaf(3.0f * c.w, t, 2.0f * c.z), t, c.y) * rdt;
         return true;
     } else {
         int j = 0;
@@ -252,7 +264,8 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
             }
         }
         if (cnt == 0) continue;  // uniform: no pixel of the tile uses this channel
-        const bool binned = cnt <= BWD_BINS;
+        // korder_cubic: four cotangents per interval, added with global atomics
+        const bool binned = GK != GK_KORDER && cnt <= BWD_BINS;
         const int64_t chan = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
         float *gy = A.gy + chan * A.gstride;
         float *ge = (GK == GK_CUBIC) ? A.ge + chan * A.gstride : nullptr;
@@ -267,6 +280,16 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                     if (nearest_index(x, A.t0, A.dt, A.nt, it) && g != 0.0f) {
                         if (binned) bin_add(by, byf, it - lo, g);
                         else unsafeAtomicAdd(gy + it, g);
+                    }
+                } else if (GK == GK_KORDER) {
+                    int j = 0;
+                    if (knot_index(x, A.t0, A.dt, A.tmax, A.nt, j) && g != 0.0f) {
+                        const float t = ((x - A.t0) - A.dt * (float)j) * A.rdt;
+                        const int64_t o = chan * A.gstride + j;
+                        unsafeAtomicAdd(A.gy + o, g);
+                        unsafeAtomicAdd(A.ge + o, g * t);
+                        unsafeAtomicAdd(A.gc2 + o, g * (t * t));
+                        unsafeAtomicAdd(A.gc3 + o, g * ((t * t) * t));
                     }
                 } else {
                     int j = 0;
@@ -356,7 +379,9 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         else
             atomicAdd(bf + i, v);
     };
-    const int narr = GK == GK_CUBIC ? 2 : 1;
+    unsigned long long *b2 = bins + 2 * nb, *b3 = bins + 3 * nb;  // korder_cubic only
+    float *b2f = reinterpret_cast<float *>(b2), *b3f = reinterpret_cast<float *>(b3);
+    const int narr = GK == GK_CUBIC ? 2 : (GK == GK_KORDER ? 4 : 1);
     for (int t = tid; t < narr * nb; t += blockDim.x) bins[t] = 0ull;
     __syncthreads();
 
@@ -381,6 +406,15 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         if (GK == GK_NEAREST) {
             int it = 0;
             if (nearest_index(x, A.t0, A.dt, A.nt, it)) bin_add(by, byf, it, g);
+        } else if (GK == GK_KORDER) {
+            int j = 0;
+            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
+            // |t| <= 1 (to rounding): every contribution stays within the bound
+            const float t = ((x - A.t0) - A.dt * (float)j) * A.rdt;
+            bin_add(by, byf, j, g);
+            bin_add(be, bef, j, g * t);
+            bin_add(b2, b2f, j, g * (t * t));
+            bin_add(b3, b3f, j, g * ((t * t) * t));
         } else {
             int j = 0;
             if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
@@ -406,11 +440,71 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
     }
     __syncthreads();
     float *gy = A.gy + chan * A.gstride;
-    float *ge = GK == GK_CUBIC ? A.ge + chan * A.gstride : nullptr;
+    float *ge = GK != GK_NEAREST && GK != GK_LINEAR ? A.ge + chan * A.gstride : nullptr;
+    auto take = [&](const unsigned long long *b, const float *bf, int t) -> float {
+        return fix ? (float)((double)(long long)b[t] * iscale) : bf[t];
+    };
     for (int t = tid; t < nb; t += blockDim.x) {
-        gy[t] = fix ? (float)((double)(long long)by[t] * iscale) : byf[t];
-        if (GK == GK_CUBIC) ge[t] = fix ? (float)((double)(long long)be[t] * iscale) : bef[t];
+        gy[t] = take(by, byf, t);
+        if (GK == GK_CUBIC || GK == GK_KORDER) ge[t] = take(be, bef, t);
+        if (GK == GK_KORDER) {
+            A.gc2[chan * A.gstride + t] = take(b2, b2f, t);
+            A.gc3[chan * A.gstride + t] = take(b3, b3f, t);
+        }
     }
+}
+
+// Adjoint of k_korder_coefs: drf[i] of one channel from the cotangents of the
+// Hermite coefficients (gc0..gc3 = d/dc.x .. d/dc.w per interval j).  With
+// c.x = y_j, c.y = d0, c.z = 3(y_j+1 - y_j) - 2 d0 - d1,
+// c.w = 2(y_j - y_j+1) + d0 + d1 and d0, d1 rows ik0, ik0+1 of S applied to
+// the window y[iy0 .. iy0+k) (placement of pycbf/gpu/__engines__.cu:314-328),
+//   drf[i] = gc0[i] - 3 gc2[i] + 2 gc3[i]  (i < nt-1; gc0 alone at nt-1)
+//          + 3 gc2[i-1] - 2 gc3[i-1]
+//          + sum over windows holding i of S[ik0][i-iy0] gd0_j + S[ik0+1][i-iy0] gd1_j
+// with gd0 = gc1 - 2 gc2 + gc3 and gd1 = gc3 - gc2.  The windows holding i
+// belong to intervals j within k of i.  One lane per (channel, sample), S in
+// LDS; a gather, so no atomics.
+__global__ __launch_bounds__(256) void k_korder_adjoint(const float *__restrict__ gc0,
+                                                        const float *__restrict__ gc1,
+                                                        const float *__restrict__ gc2,
+                                                        const float *__restrict__ gc3, int gstride,
+                                                        int64_t nch, int nt,
+                                                        const float *__restrict__ S, int k,
+                                                        float *__restrict__ grf)
+{
+    extern __shared__ float Ss[];
+    for (int q = threadIdx.x; q < k * k; q += blockDim.x) Ss[q] = S[q];
+    __syncthreads();
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nch * nt) return;
+    const int64_t ch = g / nt;
+    const int i = (int)(g - ch * nt);
+    const int64_t rb = ch * (int64_t)gstride;
+    float acc = gc0[rb + i];
+    if (i < nt - 1) acc += 2.0f * gc3[rb + i] - 3.0f * gc2[rb + i];
+    if (i >= 1) acc += 3.0f * gc2[rb + i - 1] - 2.0f * gc3[rb + i - 1];
+    const int jlo = max(0, i - k), jhi = min(nt - 2, i + k);
+    for (int j = jlo; j <= jhi; ++j) {
+        int ik0, iy0;
+        if (j < k / 2) {
+            ik0 = j;
+            iy0 = 0;
+        } else if (j > nt - k / 2 - 1) {
+            ik0 = k - nt + j;
+            iy0 = nt - k;
+        } else {
+            ik0 = (k - 1) / 2;
+            iy0 = j - (k - 1) / 2;
+        }
+        const int e = i - iy0;
+        if (e < 0 || e >= k) continue;
+        const float c1 = gc1[rb + j], c2 = gc2[rb + j], c3 = gc3[rb + j];
+        const float gd0 = c1 - 2.0f * c2 + c3, gd1 = c3 - c2;
+        acc = __builtin_fmaf(Ss[ik0 * k + e], gd0, acc);
+        acc = __builtin_fmaf(Ss[(ik0 + 1) * k + e], gd1, acc);
+    }
+    grf[g] = acc;
 }
 
 // drf[k] = gy[k] + (u[k-1] - 2u[k] + u[k+1]) with T u = ge[1..n-2], u = 0
@@ -753,7 +847,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         const int64_t nch = (int64_t)a.nfr * a.ntx * a.nrx;
         ProfScope ps("das_bwd_rf", s);
         // one workgroup per channel while the whole trace's bins fit in LDS
-        const size_t ch_lds = (size_t)(GK == GK_CUBIC ? 2 : 1) * a.gstride * 8;
+        const size_t ch_lds =
+            (size_t)(GK == GK_CUBIC ? 2 : (GK == GK_KORDER ? 4 : 1)) * a.gstride * 8;
         int dev = 0, max_lds = 0;
         if (int rc = check(hipGetDevice(&dev), "device")) return rc;
         if (int rc = check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
@@ -774,10 +869,15 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         if (per_channel) {
             hipLaunchKernelGGL(k_das_bwd_rf_ch<GK>, dim3((unsigned)nch), dim3(1024), ch_lds, s, a);
         } else {
-            if (GK == GK_CUBIC) {
+            if (GK == GK_CUBIC || GK == GK_KORDER) {
                 if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
                 if (int rc = fill_zero(a.ge, nch * a.gstride, s)) return rc;
-            } else {
+            }
+            if (GK == GK_KORDER) {
+                if (int rc = fill_zero(a.gc2, nch * a.gstride, s)) return rc;
+                if (int rc = fill_zero(a.gc3, nch * a.gstride, s)) return rc;
+            }
+            if (GK == GK_NEAREST || GK == GK_LINEAR) {
                 if (int rc = fill_zero(grf, nch * a.nt, s)) return rc;
             }
             hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
@@ -797,6 +897,13 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
                                    0, s, a.gy, a.ge, a.gstride, nch, a.nt, grf);
             if (int rc = check_launch("spline_adjoint")) return rc;
         }
+        if (GK == GK_KORDER) {
+            const int64_t n = nch * a.nt;
+            hipLaunchKernelGGL(k_korder_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                               (size_t)a.k * a.k * sizeof(float), s, a.gy, a.ge, a.gc2, a.gc3,
+                               a.gstride, nch, a.nt, a.S, a.k, grf);
+            if (int rc = check_launch("korder_adjoint")) return rc;
+        }
     }
     return 0;
 }
@@ -807,6 +914,7 @@ int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, floa
     switch (gk) {
     case GK_NEAREST: return launch_bwd_t<GK_NEAREST>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     case GK_LINEAR: return launch_bwd_t<GK_LINEAR>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    case GK_KORDER: return launch_bwd_t<GK_KORDER>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     default: return launch_bwd_t<GK_CUBIC>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     }
 }

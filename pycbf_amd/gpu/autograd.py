@@ -2,7 +2,7 @@
 pycbf_das_tabbed and whose backward is pycbf_das_tabbed_bwd (hand-written
 adjoint kernels: transposed gather into LDS bins for the channel data, a
 per-(element, pixel) pass for the delays and apodisations, and the adjoint of
-the natural-spline prefilter).
+the natural-spline prefilter or of the k-order Hermite coefficients).
 
 The forward is pycbf's Tabbed DAS (pycbf/cpu/pycbf.c:179-271 with the GPU
 layouts of pycbf/gpu/__engines__.cu:813-864); pycbf itself has no gradients.
@@ -15,7 +15,7 @@ import ctypes
 from ..base import BeamformerException
 from . import _engine as E
 
-DIFF_KINDS = ("nearest", "linear", "cubic")
+DIFF_KINDS = ("nearest", "linear", "cubic", "korder_cubic")
 
 
 def _torch():
@@ -41,13 +41,27 @@ def _out_shape(sumtype, nfr, ntx, nrx, np_):
 class _TabbedDAS:
     """Container of the non-tensor arguments (hashable-free, kept on ctx)."""
 
-    def __init__(self, t0, dt, kind, sumtype, thresh):
+    def __init__(self, t0, dt, kind, sumtype, thresh, k=None):
         if kind not in DIFF_KINDS:
             raise BeamformerException(f"differentiable kinds are {DIFF_KINDS}, got {kind!r}")
         if sumtype not in E.SUMTYPE:
             raise BeamformerException(f"unknown sumtype {sumtype!r}")
         self.t0, self.dt, self.kind, self.sumtype = float(t0), float(dt), kind, sumtype
         self.thresh = float(thresh)
+        self.param, self.S = 1, None
+        if kind == "korder_cubic":
+            from .core import make_S_by_k
+            k = 8 if k is None else int(k)
+            if k < 4 or k % 2:
+                raise BeamformerException("'k' for korder_cubic must be at least four and even")
+            self.param = k
+            self.S_host = make_S_by_k(k).astype("float32")
+
+    def S_on(self, device):
+        """The (k, k) operator on ``device`` (korder_cubic), else None."""
+        if self.S is None and self.param > 1:
+            self.S = _torch().as_tensor(self.S_host, device=device)
+        return self.S
 
 
 def _das_function():
@@ -64,11 +78,12 @@ def _das_function():
             ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
             out = torch.empty(_out_shape(cfg.sumtype, nfr, ntx, nrx, np_), dtype=torch.float32,
                               device=rf.device)
+            S = cfg.S_on(rf.device)
             rc = E.lib.pycbf_das_tabbed(ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx),
                                         E.ptr(apodtx), E.ptr(taurx), E.ptr(apodrx), None, np_,
-                                        kind, 1, None, cfg.thresh, E.SUMTYPE[cfg.sumtype],
-                                        E.ptr(out), E.ptr(ws), ws.numel(),
-                                        E.stream_of(rf.device))
+                                        kind, cfg.param, E.ptr(S), cfg.thresh,
+                                        E.SUMTYPE[cfg.sumtype], E.ptr(out), E.ptr(ws),
+                                        ws.numel(), E.stream_of(rf.device))
             E.check(rc, "pycbf_das_tabbed")
             ctx.save_for_backward(rf, tautx, apodtx, taurx, apodrx)
             ctx.cfg = cfg
@@ -91,7 +106,8 @@ def _das_function():
             ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
             rc = E.lib.pycbf_das_tabbed_bwd(
                 ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx), E.ptr(apodtx), E.ptr(taurx),
-                E.ptr(apodrx), np_, kind, cfg.thresh, E.SUMTYPE[cfg.sumtype], E.ptr(gout),
+                E.ptr(apodrx), np_, kind, cfg.param, E.ptr(cfg.S_on(rf.device)), cfg.thresh,
+                E.SUMTYPE[cfg.sumtype], E.ptr(gout),
                 E.ptr(grf), E.ptr(gttx), E.ptr(gatx), E.ptr(gtrx), E.ptr(garx), E.ptr(ws),
                 ws.numel(), E.stream_of(rf.device))
             E.check(rc, "pycbf_das_tabbed_bwd")
@@ -104,19 +120,20 @@ _FN = None
 
 
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype="tx_and_rx",
-               thresh=0.0):
+               thresh=0.0, k=None):
     """Differentiable Tabbed DAS on the GPU.
 
     rf (ntx, nrx, nt) or (nfr, ntx, nrx, nt); tautx / apodtx (ntx, np);
     taurx / apodrx (nrx, np); all float32 CUDA tensors, any of which may
-    require grad.  ``interp`` is 'nearest', 'linear' or 'cubic' (natural
-    spline); the output has the layout of TabbedBeamformer for ``sumtype``.
+    require grad.  ``interp`` is 'nearest', 'linear', 'cubic' (natural
+    spline) or 'korder_cubic' (``k``, default 8); the output has the layout
+    of TabbedBeamformer for ``sumtype``.
     ``thresh`` (default 0: every sample) drops samples with
     |apodtx * apodrx| < thresh in the forward and the adjoint alike."""
     global _FN
     if _FN is None:
         _FN = _das_function()
-    cfg = _TabbedDAS(t0, dt, interp, sumtype, thresh)
+    cfg = _TabbedDAS(t0, dt, interp, sumtype, thresh, k)
     squeeze = rf.dim() == 3
     rf4 = rf[None] if squeeze else rf
     if rf4.dim() != 4:

@@ -32,13 +32,35 @@ def test_natural_y2_matches_scipy():
     np.testing.assert_allclose(got, ref, atol=1e-9)
 
 
-@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic"])
+@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
 def test_forward_agrees_with_float32_oracle(kind):
+    from oracle import numerics
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(2)
     f32 = [np.float32(a) for a in (rf, tautx, apodtx, taurx, apodrx)]
-    got = diff64.das_tabbed(*(torch.as_tensor(a) for a in f32), t0, dt, kind).numpy()[0]
-    ref = oracle.das_tabbed(f32[0][0], t0, dt, *f32[1:], kind=kind, thresh=0.0)
+    S = numerics.make_S(8).astype(np.float32) if kind == "korder_cubic" else None
+    got = diff64.das_tabbed(*(torch.as_tensor(a) for a in f32), t0, dt, kind,
+                            S=None if S is None else S.astype(np.float64)).numpy()[0]
+    ref = oracle.das_tabbed(f32[0][0], t0, dt, *f32[1:], kind=kind, thresh=0.0, S=S)
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_korder_coefs_reproduce_samples_and_cubics():
+    """The Hermite pieces pass through the samples (c0 = y_j, c0 + c1 + c2 + c3
+    = y_j+1), and a cubic polynomial sampled on the grid is reproduced exactly
+    away from the ends (the not-a-knot derivatives of a cubic are exact)."""
+    from oracle import numerics
+    k, n = 8, 40
+    y = torch.as_tensor(np.random.default_rng(9).standard_normal(n))
+    C = diff64.korder_coefs(y, numerics.make_S(k))
+    torch.testing.assert_close(C[:, 0], y, rtol=0, atol=1e-12)
+    torch.testing.assert_close(C[:-1].sum(-1), y[1:], rtol=0, atol=1e-12)
+    xs = np.arange(n, dtype=np.float64)
+    poly = lambda x: 0.3 - 1.2 * x + 0.05 * x ** 2 - 0.002 * x ** 3  # noqa: E731
+    C = diff64.korder_coefs(torch.as_tensor(poly(xs)), numerics.make_S(k))
+    for j in (0, 5, 17, n - 3):
+        for t in (0.25, 0.5, 0.9):
+            v = C[j, 0] + t * (C[j, 1] + t * (C[j, 2] + t * C[j, 3]))
+            assert abs(float(v) - poly(j + t)) <= 1e-9 * (1 + abs(poly(j + t)))
 
 
 def test_linear_in_channel_data():
@@ -51,18 +73,20 @@ def test_linear_in_channel_data():
                                atol=1e-12)
 
 
-@pytest.mark.parametrize("kind", ["linear", "cubic"])
+@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic"])
 def test_autograd_matches_central_differences(kind):
+    from oracle import numerics
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(5)
+    S = numerics.make_S(8) if kind == "korder_cubic" else None
     args = [torch.as_tensor(a, dtype=torch.float64).requires_grad_()
             for a in (rf, tautx, apodtx, taurx, apodrx)]
     gout = torch.as_tensor(np.random.default_rng(6).standard_normal((1, 40)))
-    loss = (diff64.das_tabbed(*args, t0, dt, kind, x32=False) * gout).sum()
+    loss = (diff64.das_tabbed(*args, t0, dt, kind, x32=False, S=S) * gout).sum()
     grads = torch.autograd.grad(loss, args)
 
     def L(vals):
         with torch.no_grad():
-            return float((diff64.das_tabbed(*vals, t0, dt, kind, x32=False) * gout).sum())
+            return float((diff64.das_tabbed(*vals, t0, dt, kind, x32=False, S=S) * gout).sum())
 
     rng = np.random.default_rng(7)
     for ai, h in ((0, 1e-4), (1, 1e-13), (2, 1e-6), (3, 1e-13), (4, 1e-6)):

@@ -35,11 +35,21 @@ def _case(seed, ntx=3, nrx=8, nt=160, np_=300, nfr=1, discrete=False):
     return rf, tautx, apodtx, taurx, apodrx, t0, dt
 
 
+KORDER_K = 8
+
+
+def _S64():
+    """The k-order operator as the GPU uses it (float32), in float64."""
+    from oracle import numerics
+    return numerics.make_S(KORDER_K).astype(np.float32).astype(np.float64)
+
+
 def _grads_gpu(arrs, t0, dt, kind, sumtype, thresh, gout):
     import torch
     from pycbf_amd.gpu.autograd import das_tabbed
     t = [torch.as_tensor(a, device="cuda").requires_grad_() for a in arrs]
-    out = das_tabbed(*t, t0, dt, kind, sumtype, thresh)
+    out = das_tabbed(*t, t0, dt, kind, sumtype, thresh,
+                     k=KORDER_K if kind == "korder_cubic" else None)
     g = torch.as_tensor(gout, device="cuda", dtype=torch.float32)
     (out * g).sum().backward()
     return out.detach().cpu().numpy(), [x.grad.cpu().numpy() for x in t]
@@ -49,7 +59,8 @@ def _grads_ref(arrs, t0, dt, kind, sumtype, thresh, gout, x32=True):
     import torch
     from oracle import diff64
     t = [torch.as_tensor(a, dtype=torch.float64).requires_grad_() for a in arrs]
-    out = diff64.das_tabbed(*t, t0, dt, kind, sumtype, thresh, x32=x32)
+    out = diff64.das_tabbed(*t, t0, dt, kind, sumtype, thresh, x32=x32,
+                            S=_S64() if kind == "korder_cubic" else None)
     (out * torch.as_tensor(gout, dtype=torch.float64)).sum().backward()
     return out.detach().numpy(), [np.zeros(x.shape) if x.grad is None else x.grad.numpy()
                                   for x in t]
@@ -59,7 +70,7 @@ NAMES = ["rf", "tautx", "apodtx", "taurx", "apodrx"]
 
 
 @pytest.mark.parametrize("sumtype", SUMTYPES)
-@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic"])
+@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
 def test_adjoint_matches_fp64_autograd(kind, sumtype):
     print(f"adjoint {kind} {sumtype}", flush=True)
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(11, nfr=2)
@@ -78,7 +89,7 @@ def test_adjoint_matches_fp64_autograd(kind, sumtype):
         assert err <= 1e-4 * np.max(np.abs(b)), f"{kind} {sumtype} d{name}: {err:.3e}"
 
 
-@pytest.mark.parametrize("kind", ["linear", "cubic"])
+@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic"])
 def test_adjoint_threshold_matches_forward_gate(kind):
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(13, discrete=True)
     arrs = [rf, tautx, apodtx, taurx, apodrx]
@@ -151,7 +162,7 @@ def test_cubic_rf_adjoint_trace_lengths(nt):
         assert float(np.abs(g[0] - g1[0]).max()) <= 1e-6 * peak
 
 
-@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic"])
+@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
 def test_rf_adjoint_transpose_identity_full_size(kind):
     """<A rf, G> = <rf, A^T G> for the 31 x 128 x 4096, 256 x 256 workload."""
     import torch
@@ -166,9 +177,39 @@ def test_rf_adjoint_transpose_identity_full_size(kind):
     g.manual_seed(18)
     rf = torch.randn((31, 128, 4096), generator=g, device="cuda", requires_grad=True)
     G = torch.randn((256 * 256,), generator=g, device="cuda")
-    out = das_tabbed(rf, *tabs, 0.0, 1.0 / 40e6, kind, "tx_and_rx", 0.0)
+    out = das_tabbed(rf, *tabs, 0.0, 1.0 / 40e6, kind, "tx_and_rx", 0.0,
+                     k=KORDER_K if kind == "korder_cubic" else None)
     (grf,) = torch.autograd.grad((out * G).sum(), [rf])
     lhs = float((out.double() * G.double()).sum())
     rhs = float((rf.detach().double() * grf.double()).sum())
     assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1e-30), (lhs, rhs)
     assert float(grf.abs().max()) > 0
+
+
+@pytest.mark.parametrize("nt,k", [(8, 8), (9, 8), (160, 4), (300, 16), (4097, 8), (4100, 8)])
+def test_korder_adjoint_window_edges_and_long_traces(nt, k):
+    """korder_cubic adjoint against FP64 autograd for windows clamped at both
+    ends (nt close to k) and for traces past one workgroup's LDS bins (the
+    per-tile path with global atomics beyond 4097 samples)."""
+    import torch
+    from oracle import diff64, numerics
+    from pycbf_amd.gpu.autograd import das_tabbed
+    print(f"korder adjoint nt={nt} k={k}", flush=True)
+    rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(31 + nt, ntx=2, nrx=5, nt=nt, np_=260)
+    arrs = [rf, tautx, apodtx, taurx, apodrx]
+    gout = np.random.default_rng(32).standard_normal((1, 260)).astype(np.float32)
+    t = [torch.as_tensor(a, device="cuda").requires_grad_() for a in arrs]
+    out = das_tabbed(*t, t0, dt, "korder_cubic", "tx_and_rx", 0.0, k=k)
+    (out * torch.as_tensor(gout, device="cuda")).sum().backward()
+    S = numerics.make_S(k).astype(np.float32).astype(np.float64)
+    r = [torch.as_tensor(a, dtype=torch.float64).requires_grad_() for a in arrs]
+    ref = diff64.das_tabbed(*r, t0, dt, "korder_cubic", "tx_and_rx", 0.0, S=S)
+    (ref * torch.as_tensor(gout, dtype=torch.float64)).sum().backward()
+    o, ro = out.detach().cpu().numpy(), ref.detach().numpy()
+    # float32 sample positions x <= t0 + nt dt carry a relative weight error of
+    # 2^-23 (t0/dt + nt) that the float64 reference does not have
+    tol = max(1e-4, 2.0 ** -23 * (t0 / dt + nt))
+    assert np.max(np.abs(o - ro)) <= tol * np.max(np.abs(ro))
+    for name, a, b in zip(NAMES, t, r):
+        ga, gb = a.grad.cpu().numpy(), b.grad.numpy()
+        assert np.max(np.abs(ga - gb)) <= tol * np.max(np.abs(gb)), name
