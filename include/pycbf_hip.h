@@ -207,7 +207,8 @@ int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int
 
 /* Bytes of workspace pycbf_das_tabbed_bwd needs (cubic: spline table and
  * cotangent bins of every channel; korder_cubic: Hermite table and the
- * cotangents of its four coefficients; all kinds: per-tile delay ranges). */
+ * cotangents of its four coefficients, as for akima and makima; all kinds:
+ * per-tile delay ranges). */
 size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
                                      int interp_kind);
 
@@ -221,8 +222,10 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
  * (param = usf = 1; its delay gradient is 0), linear, cubic (natural spline:
  * the adjoint of the spline prefilter is applied to grf) or korder_cubic
  * (param = k, S the (k, k) operator as for pycbf_das_tabbed; the adjoint of
- * the Hermite coefficient construction is applied to grf).  Samples with
- * |apodtx*apodrx| < thresh contribute nothing, as in the forward. */
+ * the Hermite coefficient construction is applied to grf), akima or makima
+ * (the adjoint through the Akima derivatives, whose weights |.| differentiate
+ * to sign(.) with sign(0) = 0).  Samples with |apodtx*apodrx| < thresh
+ * contribute nothing, as in the forward. */
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          const float *tautx, const float *apodtx, const float *taurx,
                          const float *apodrx, int64_t np, int interp_kind, int param,

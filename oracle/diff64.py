@@ -17,6 +17,10 @@ delays and apodisations:
     Hermite coefficients of pycbf/gpu/__engines__.cu:292-355 (end values y_j, y_j+1 and
     derivatives d0, d1 = rows ik0, ik0+1 of S applied to the k-sample window at iy0,
     placed as :314-328); interval n-1 is the constant y_{n-1}
+  akima / makima: t = (x - (t0 + dt j)) / dt, the same Hermite form with the
+    per-sample Akima derivatives (Akima's weights |m_i+1 - m_i|, |m_i-1 - m_i-2|,
+    makima adding |m_i+1 + m_i| / 2, |m_i-1 + m_i-2| / 2; two extra slopes at each
+    end; (m_i-1 + m_i) / 2 where both weights vanish)
 with y_n = y2_n = 0 (the zero entry past the trace) and y2 the natural
 spline's second derivatives (y2_0 = y2_{n-1} = 0), solved exactly in float64.
 
@@ -69,6 +73,31 @@ def korder_coefs(rf: torch.Tensor, S) -> torch.Tensor:
     return torch.cat([c, last], dim=-2)
 
 
+def akima_coefs(rf: torch.Tensor, modified: bool) -> torch.Tensor:
+    """Hermite coefficients (..., n, 4) of the Akima (or makima) interpolant
+    through rf (..., n) on unit spacing: c = hermite(y_j, y_j+1, d_j, d_j+1),
+    c = (y_{n-1}, 0, 0, 0) for the last interval."""
+    m = rf[..., 1:] - rf[..., :-1]                              # (..., n-1)
+    lo = torch.stack([3.0 * m[..., 0] - 2.0 * m[..., 1], 2.0 * m[..., 0] - m[..., 1]], -1)
+    hi = torch.stack([2.0 * m[..., -1] - m[..., -2], 3.0 * m[..., -1] - 2.0 * m[..., -2]], -1)
+    mp = torch.cat([lo, m, hi], -1)                             # slopes -2 .. n, (..., n+3)
+    mm2, mm1, mp0, mp1 = mp[..., :-3], mp[..., 1:-2], mp[..., 2:-1], mp[..., 3:]
+    w1 = (mp1 - mp0).abs()
+    w2 = (mm1 - mm2).abs()
+    if modified:
+        w1 = w1 + 0.5 * (mp1 + mp0).abs()
+        w2 = w2 + 0.5 * (mm1 + mm2).abs()
+    ws = w1 + w2
+    pos = ws > 0
+    safe = torch.where(pos, ws, torch.ones_like(ws))
+    d = torch.where(pos, (w1 * mm1 + w2 * mp0) / safe, 0.5 * (mm1 + mp0))   # (..., n)
+    y0, y1, d0, d1 = rf[..., :-1], rf[..., 1:], d[..., :-1], d[..., 1:]
+    c = torch.stack([y0, d0, 3.0 * (y1 - y0) - 2.0 * d0 - d1, 2.0 * (y0 - y1) + d0 + d1], -1)
+    z = torch.zeros_like(rf[..., -1:])
+    last = torch.stack([rf[..., -1:], z, z, z], -1)
+    return torch.cat([c, last], dim=-2)
+
+
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="tx_and_rx",
                thresh=0.0, x32=True, S=None):
     """rf (nfr, ntx, nrx, n); tables (ntx, np) / (nrx, np); returns float64
@@ -112,11 +141,15 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="
         d = qd + 0.5
         inside = ((d > -1.0) & (d < n)).to(torch.float64)
         v = take(y, torch.trunc(d).clamp(0, n - 1).long())
-    elif kind == "korder_cubic":
+    elif kind in ("korder_cubic", "akima", "makima"):
         inside = (lo_ok & hi_ok).to(torch.float64)
         j = torch.floor(qd).clamp(0, n - 1).long()
-        t = ((x - t0) - dt * j.to(torch.float64)) / dt
-        C = korder_coefs(rf, S)                                # (nfr, ntx, nrx, n, 4)
+        if kind == "korder_cubic":
+            t = ((x - t0) - dt * j.to(torch.float64)) / dt
+            C = korder_coefs(rf, S)                            # (nfr, ntx, nrx, n, 4)
+        else:
+            t = (x - (t0 + dt * j.to(torch.float64))) / dt
+            C = akima_coefs(rf, kind == "makima")
         cj = torch.gather(C, 3, j[None, ..., None].expand(nfr, *j.shape, 4))
         v = cj[..., 0] + t * (cj[..., 1] + t * (cj[..., 2] + t * cj[..., 3]))
     else:

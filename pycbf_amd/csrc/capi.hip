@@ -528,7 +528,8 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         b.coef = align_up((size_t)nch * (size_t)(nt + 1) * 8, 256);
         b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
         b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
-    } else if (kind == PYCBF_INTERP_KORDER_CUBIC) {
+    } else if (kind == PYCBF_INTERP_KORDER_CUBIC || kind == PYCBF_INTERP_AKIMA ||
+               kind == PYCBF_INTERP_MAKIMA) {
         b.coef = align_up((size_t)nch * (size_t)nt * 16, 256);
         b.gy = b.ge = b.gc2 = b.gc3 = align_up((size_t)nch * (size_t)nt * 4, 256);
     }
@@ -551,9 +552,11 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          float *gapodrx, void *workspace, size_t workspace_bytes, void *stream)
 {
     if (interp_kind != PYCBF_INTERP_NEAREST && interp_kind != PYCBF_INTERP_LINEAR &&
-        interp_kind != PYCBF_INTERP_CUBIC && interp_kind != PYCBF_INTERP_KORDER_CUBIC)
+        interp_kind != PYCBF_INTERP_CUBIC && interp_kind != PYCBF_INTERP_KORDER_CUBIC &&
+        interp_kind != PYCBF_INTERP_AKIMA && interp_kind != PYCBF_INTERP_MAKIMA)
         return fail(PYCBF_EINVAL,
-                    "the adjoint supports nearest, linear, cubic and korder_cubic (got %d)",
+                    "the adjoint supports nearest, linear, cubic, korder_cubic, akima and makima "
+                    "(got %d)",
                     interp_kind);
     if (interp_kind == PYCBF_INTERP_NEAREST && param != 1)
         return fail(PYCBF_EINVAL, "the nearest adjoint needs usf = 1");
@@ -614,6 +617,17 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         b.k = param;
         if ((rc = launch_korder_coefs(rf, nch, nt, S, param, coef, s))) return rc;
         b.src = (const float *)coef;
+    } else if (interp_kind == PYCBF_INTERP_AKIMA || interp_kind == PYCBF_INTERP_MAKIMA) {
+        float4 *coef = (float4 *)cv.take(plan.coef);
+        b.gy = (float *)cv.take(plan.gy);
+        b.ge = (float *)cv.take(plan.ge);
+        b.gc2 = (float *)cv.take(plan.gc2);
+        b.gc3 = (float *)cv.take(plan.gc3);
+        b.gstride = nt;
+        b.rf = rf;
+        b.modified = interp_kind == PYCBF_INTERP_MAKIMA;
+        if ((rc = launch_akima_coefs(rf, nch, nt, b.modified, coef, s))) return rc;
+        b.src = (const float *)coef;
     } else {
         b.gy = grf;
         b.gstride = nt;
@@ -634,7 +648,8 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
     const int gk = interp_kind == PYCBF_INTERP_NEAREST  ? GK_NEAREST
                    : interp_kind == PYCBF_INTERP_LINEAR ? GK_LINEAR
                    : interp_kind == PYCBF_INTERP_CUBIC  ? GK_CUBIC
-                                                        : GK_KORDER;
+                   : interp_kind == PYCBF_INTERP_KORDER_CUBIC ? GK_KORDER
+                                                              : GK_AKIMA;
     return launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
 }
 

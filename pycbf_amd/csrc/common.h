@@ -126,12 +126,15 @@ struct BwdArgs {
     const float *gout;
     const float *bounds;  // (ntiles, ntx + nrx, 2) delay ranges of contributing pixels
     float *gtau, *gapod;  // outer-axis gradients (may be null)
-    float *gy, *ge;       // channel-data cotangent bins (ge: cubic only); korder_cubic:
-                          // cotangents of the Hermite coefficients c.x (gy), c.y (ge),
-    float *gc2, *gc3;     // c.z and c.w (korder_cubic only)
+    float *gy, *ge;       // channel-data cotangent bins (ge: cubic only); Hermite kinds
+                          // (korder_cubic, akima, makima): cotangents of the
+                          // coefficients c.x (gy), c.y (ge),
+    float *gc2, *gc3;     // c.z and c.w (Hermite kinds only)
     int gstride;          // floats per channel row of gy / ge / gc2 / gc3
     const float *S;       // korder_cubic: (k, k) derivative operator
     int k;
+    int modified;         // akima (0) / makima (1)
+    const float *rf;      // akima / makima: the raw channel data (slopes are recomputed)
     unsigned *mbits;      // 3 words of workspace: max |gout|, |apodtx|, |apodrx| (bits)
 };
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,

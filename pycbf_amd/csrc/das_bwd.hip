@@ -26,11 +26,12 @@
 // e = dt^2/6 y2, and k_spline_adjoint(_seg) adds the transposed spline solve
 //   dy += D2(T^-1 de),  T = tridiag(1, 4, 1),  D2 the second difference,
 // which is the adjoint of e = T^-1 D2 y (the natural spline with y2 = 0 at
-// both ends).  For korder_cubic the bins hold the cotangents of the four
-// Hermite coefficients of each interval (the value is c.x + c.y t + c.z t^2
-// + c.w t^3), and k_korder_adjoint transposes their construction from the
-// samples (k_korder_coefs: end values and S-derivatives over a k-sample
-// window).
+// both ends).  For the Hermite kinds (korder_cubic, akima, makima) the bins
+// hold the cotangents of the four coefficients of each interval (the value
+// is c.x + c.y t + c.z t^2 + c.w t^3), and k_korder_adjoint /
+// k_akima_adjoint transpose their construction from the samples
+// (k_korder_coefs: end values and S-derivatives over a k-sample window;
+// k_akima_coefs: end values and Akima derivatives).
 #include <math.h>
 
 #include <algorithm>
@@ -58,10 +59,12 @@ __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t
         v = src[row * n + it];
         d = 0.0f;
         return true;
-    } else if (GK == GK_KORDER) {
+    } else if (GK == GK_KORDER || GK == GK_AKIMA) {
         int j = 0;
         if (!knot_index(x, t0, dt, tmax, n, j)) return false;
-        const float t = ((x - t0) - dt * (float)j) * rdt;
+        // the forward's roundings of t (korder_cubic / akima)
+        const float t = (GK == GK_KORDER) ? ((x - t0) - dt * (float)j) * rdt
+                                          : (x - (t0 + dt * (float)j)) * rdt;
         const float4 c = reinterpret_cast<const float4 *>(src)[row * n + j];
         v = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, t, c.z), t, c.y), t, c.x);
         d = __builtin_fmaf(__builtin_fmaf(3.0f * c.w, t, 2.0f * c.z), t, c.y) * rdt;
@@ -264,8 +267,8 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
             }
         }
         if (cnt == 0) continue;  // uniform: no pixel of the tile uses this channel
-        // korder_cubic: four cotangents per interval, added with global atomics
-        const bool binned = GK != GK_KORDER && cnt <= BWD_BINS;
+        // Hermite kinds: four cotangents per interval, added with global atomics
+        const bool binned = GK != GK_KORDER && GK != GK_AKIMA && cnt <= BWD_BINS;
         const int64_t chan = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
         float *gy = A.gy + chan * A.gstride;
         float *ge = (GK == GK_CUBIC) ? A.ge + chan * A.gstride : nullptr;
@@ -281,10 +284,12 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                         if (binned) bin_add(by, byf, it - lo, g);
                         else unsafeAtomicAdd(gy + it, g);
                     }
-                } else if (GK == GK_KORDER) {
+                } else if (GK == GK_KORDER || GK == GK_AKIMA) {
                     int j = 0;
                     if (knot_index(x, A.t0, A.dt, A.tmax, A.nt, j) && g != 0.0f) {
-                        const float t = ((x - A.t0) - A.dt * (float)j) * A.rdt;
+                        const float t = (GK == GK_KORDER)
+                                            ? ((x - A.t0) - A.dt * (float)j) * A.rdt
+                                            : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
                         const int64_t o = chan * A.gstride + j;
                         unsafeAtomicAdd(A.gy + o, g);
                         unsafeAtomicAdd(A.ge + o, g * t);
@@ -379,9 +384,10 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         else
             atomicAdd(bf + i, v);
     };
-    unsigned long long *b2 = bins + 2 * nb, *b3 = bins + 3 * nb;  // korder_cubic only
+    constexpr bool HERM = GK == GK_KORDER || GK == GK_AKIMA;
+    unsigned long long *b2 = bins + 2 * nb, *b3 = bins + 3 * nb;  // Hermite kinds only
     float *b2f = reinterpret_cast<float *>(b2), *b3f = reinterpret_cast<float *>(b3);
-    const int narr = GK == GK_CUBIC ? 2 : (GK == GK_KORDER ? 4 : 1);
+    const int narr = GK == GK_CUBIC ? 2 : (HERM ? 4 : 1);
     for (int t = tid; t < narr * nb; t += blockDim.x) bins[t] = 0ull;
     __syncthreads();
 
@@ -406,11 +412,12 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         if (GK == GK_NEAREST) {
             int it = 0;
             if (nearest_index(x, A.t0, A.dt, A.nt, it)) bin_add(by, byf, it, g);
-        } else if (GK == GK_KORDER) {
+        } else if (HERM) {
             int j = 0;
             if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
             // |t| <= 1 (to rounding): every contribution stays within the bound
-            const float t = ((x - A.t0) - A.dt * (float)j) * A.rdt;
+            const float t = (GK == GK_KORDER) ? ((x - A.t0) - A.dt * (float)j) * A.rdt
+                                              : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
             bin_add(by, byf, j, g);
             bin_add(be, bef, j, g * t);
             bin_add(b2, b2f, j, g * (t * t));
@@ -446,8 +453,8 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
     };
     for (int t = tid; t < nb; t += blockDim.x) {
         gy[t] = take(by, byf, t);
-        if (GK == GK_CUBIC || GK == GK_KORDER) ge[t] = take(be, bef, t);
-        if (GK == GK_KORDER) {
+        if (GK == GK_CUBIC || HERM) ge[t] = take(be, bef, t);
+        if (HERM) {
             A.gc2[chan * A.gstride + t] = take(b2, b2f, t);
             A.gc3[chan * A.gstride + t] = take(b3, b3f, t);
         }
@@ -504,6 +511,136 @@ __global__ __launch_bounds__(256) void k_korder_adjoint(const float *__restrict_
         acc = __builtin_fmaf(Ss[ik0 * k + e], gd0, acc);
         acc = __builtin_fmaf(Ss[(ik0 + 1) * k + e], gd1, acc);
     }
+    grf[g] = acc;
+}
+
+namespace {
+// slope m_v of y (v in [-2, n]), with Akima's two-point extension at each
+// end: m_-1 = 2m_0 - m_1, m_-2 = 3m_0 - 2m_1, m_n-1 = 2m_n-2 - m_n-3,
+// m_n = 3m_n-2 - 2m_n-3 (the slopes of k_akima_coefs)
+__device__ __forceinline__ float ak_slope(const float *y, int n, int v)
+{
+    if (v >= 0 && v <= n - 2) return y[v + 1] - y[v];
+    if (v < 0) {
+        const float m0 = y[1] - y[0], m1 = y[2] - y[1];
+        return (v == -1) ? (2.0f * m0 - m1) : (3.0f * m0 - 2.0f * m1);
+    }
+    const float a = y[n - 1] - y[n - 2], b = y[n - 2] - y[n - 3];
+    return (v == n - 1) ? (2.0f * a - b) : (3.0f * a - 2.0f * b);
+}
+
+__device__ __forceinline__ float sgnf(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+
+// partial derivatives of the Akima / makima derivative d_i with respect to
+// the slopes m_i-2, m_i-1, m_i, m_i+1 (d[0..3]).  d_i = (w1 m_i-1 + w2 m_i) /
+// (w1 + w2) with w1 = |m_i+1 - m_i| (+ |m_i+1 + m_i| / 2 for makima), w2 =
+// |m_i-1 - m_i-2| (+ |m_i-1 + m_i-2| / 2), or (m_i-1 + m_i) / 2 when both
+// weights vanish; |.| differentiates to sign(.) with sign(0) = 0.
+__device__ __forceinline__ void ak_dderiv(const float *y, int n, int i, int modified, float (&d)[4])
+{
+    const float mm2 = ak_slope(y, n, i - 2), mm1 = ak_slope(y, n, i - 1);
+    const float mp0 = ak_slope(y, n, i), mp1 = ak_slope(y, n, i + 1);
+    float w1 = fabsf(mp1 - mp0), w2 = fabsf(mm1 - mm2);
+    const float s1 = sgnf(mp1 - mp0), s2 = sgnf(mm1 - mm2);
+    float h1 = 0.0f, h2 = 0.0f;
+    if (modified) {
+        w1 += 0.5f * fabsf(mp1 + mp0);
+        w2 += 0.5f * fabsf(mm1 + mm2);
+        h1 = 0.5f * sgnf(mp1 + mp0);
+        h2 = 0.5f * sgnf(mm1 + mm2);
+    }
+    const float ws = w1 + w2;
+    if (!(ws > 0.0f)) {
+        d[0] = 0.0f;
+        d[1] = 0.5f;
+        d[2] = 0.5f;
+        d[3] = 0.0f;
+        return;
+    }
+    const float di = (w1 * mm1 + w2 * mp0) / ws;
+    const float a1 = (mm1 - di) / ws;  // d di / d w1
+    const float a2 = (mp0 - di) / ws;  // d di / d w2
+    d[0] = a2 * (h2 - s2);
+    d[1] = w1 / ws + a2 * (s2 + h2);
+    d[2] = w2 / ws + a1 * (h1 - s1);
+    d[3] = a1 * (s1 + h1);
+}
+
+// cotangent of the derivative d_i from the interval coefficients: d_i is c.y
+// of interval i (c.z, c.w also hold it) and d1 of interval i-1
+__device__ __forceinline__ float ak_gd(const float *c1, const float *c2, const float *c3, int n,
+                                       int i)
+{
+    float g = 0.0f;
+    if (i <= n - 2) g += c1[i] - 2.0f * c2[i] + c3[i];
+    if (i >= 1 && i - 1 <= n - 2) g += c3[i - 1] - c2[i - 1];
+    return g;
+}
+
+// cotangent of slope m_v (v in [-2, n]) through the derivatives d_v-1 .. d_v+2
+__device__ __forceinline__ float ak_gmv(const float *y, const float *c1, const float *c2,
+                                        const float *c3, int n, int modified, int v)
+{
+    float g = 0.0f;
+    for (int i = max(v - 1, 0); i <= min(v + 2, n - 1); ++i) {
+        float d[4];
+        ak_dderiv(y, n, i, modified, d);
+        const int e = v - i + 2;  // slope v is argument e of d_i
+        g = __builtin_fmaf(ak_gd(c1, c2, c3, n, i), d[e], g);
+    }
+    return g;
+}
+
+// cotangent of the real slope m_k (k in [0, n-2]), with the end extensions
+__device__ __forceinline__ float ak_gm(const float *y, const float *c1, const float *c2,
+                                       const float *c3, int n, int modified, int k)
+{
+    float g = ak_gmv(y, c1, c2, c3, n, modified, k);
+    if (k == 0)
+        g += 2.0f * ak_gmv(y, c1, c2, c3, n, modified, -1) +
+             3.0f * ak_gmv(y, c1, c2, c3, n, modified, -2);
+    if (k == 1)
+        g -= ak_gmv(y, c1, c2, c3, n, modified, -1) +
+             2.0f * ak_gmv(y, c1, c2, c3, n, modified, -2);
+    if (k == n - 2)
+        g += 2.0f * ak_gmv(y, c1, c2, c3, n, modified, n - 1) +
+             3.0f * ak_gmv(y, c1, c2, c3, n, modified, n);
+    if (k == n - 3)
+        g -= ak_gmv(y, c1, c2, c3, n, modified, n - 1) +
+             2.0f * ak_gmv(y, c1, c2, c3, n, modified, n);
+    return g;
+}
+}  // namespace
+
+// Adjoint of k_akima_coefs: drf[i] of one channel from the cotangents of the
+// Hermite coefficients (gc0..gc3 per interval j; c = hermite(y_j, y_j+1,
+// d_j, d_j+1) for j < nt-1, c.x = y_nt-1 on the last interval):
+//   drf[i] = gc0[i] - 3 gc2[i] + 2 gc3[i]  (i < nt-1; gc0 alone at nt-1)
+//          + 3 gc2[i-1] - 2 gc3[i-1]
+//          + gm[i-1] - gm[i]               (m_k = y_k+1 - y_k)
+// with gm the slope cotangents through the Akima derivatives.  The Akima
+// derivative is evaluated and differentiated in float32 from the samples, as
+// the forward evaluates it.  One lane per (channel, sample); a gather.
+__global__ __launch_bounds__(256) void k_akima_adjoint(const float *__restrict__ gc0,
+                                                       const float *__restrict__ gc1,
+                                                       const float *__restrict__ gc2,
+                                                       const float *__restrict__ gc3, int gstride,
+                                                       int64_t nch, int nt,
+                                                       const float *__restrict__ rf, int modified,
+                                                       float *__restrict__ grf)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nch * nt) return;
+    const int64_t ch = g / nt;
+    const int i = (int)(g - ch * nt);
+    const int64_t rb = ch * (int64_t)gstride;
+    const float *y = rf + ch * (int64_t)nt;
+    const float *c1 = gc1 + rb, *c2 = gc2 + rb, *c3 = gc3 + rb;
+    float acc = gc0[rb + i];
+    if (i < nt - 1) acc += 2.0f * c3[i] - 3.0f * c2[i];
+    if (i >= 1) acc += 3.0f * c2[i - 1] - 2.0f * c3[i - 1];
+    if (i >= 1) acc += ak_gm(y, c1, c2, c3, nt, modified, i - 1);
+    if (i <= nt - 2) acc -= ak_gm(y, c1, c2, c3, nt, modified, i);
     grf[g] = acc;
 }
 
@@ -848,7 +985,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         ProfScope ps("das_bwd_rf", s);
         // one workgroup per channel while the whole trace's bins fit in LDS
         const size_t ch_lds =
-            (size_t)(GK == GK_CUBIC ? 2 : (GK == GK_KORDER ? 4 : 1)) * a.gstride * 8;
+            (size_t)(GK == GK_CUBIC ? 2 : (GK == GK_KORDER || GK == GK_AKIMA ? 4 : 1)) *
+            a.gstride * 8;
         int dev = 0, max_lds = 0;
         if (int rc = check(hipGetDevice(&dev), "device")) return rc;
         if (int rc = check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
@@ -869,11 +1007,11 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         if (per_channel) {
             hipLaunchKernelGGL(k_das_bwd_rf_ch<GK>, dim3((unsigned)nch), dim3(1024), ch_lds, s, a);
         } else {
-            if (GK == GK_CUBIC || GK == GK_KORDER) {
+            if (GK == GK_CUBIC || GK == GK_KORDER || GK == GK_AKIMA) {
                 if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
                 if (int rc = fill_zero(a.ge, nch * a.gstride, s)) return rc;
             }
-            if (GK == GK_KORDER) {
+            if (GK == GK_KORDER || GK == GK_AKIMA) {
                 if (int rc = fill_zero(a.gc2, nch * a.gstride, s)) return rc;
                 if (int rc = fill_zero(a.gc3, nch * a.gstride, s)) return rc;
             }
@@ -904,6 +1042,13 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
                                a.gstride, nch, a.nt, a.S, a.k, grf);
             if (int rc = check_launch("korder_adjoint")) return rc;
         }
+        if (GK == GK_AKIMA) {
+            const int64_t n = nch * a.nt;
+            hipLaunchKernelGGL(k_akima_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                               s, a.gy, a.ge, a.gc2, a.gc3, a.gstride, nch, a.nt, a.rf,
+                               a.modified, grf);
+            if (int rc = check_launch("akima_adjoint")) return rc;
+        }
     }
     return 0;
 }
@@ -915,6 +1060,7 @@ int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, floa
     case GK_NEAREST: return launch_bwd_t<GK_NEAREST>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     case GK_LINEAR: return launch_bwd_t<GK_LINEAR>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     case GK_KORDER: return launch_bwd_t<GK_KORDER>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    case GK_AKIMA: return launch_bwd_t<GK_AKIMA>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     default: return launch_bwd_t<GK_CUBIC>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
     }
 }

@@ -2,7 +2,8 @@
 pycbf_das_tabbed and whose backward is pycbf_das_tabbed_bwd (hand-written
 adjoint kernels: transposed gather into LDS bins for the channel data, a
 per-(element, pixel) pass for the delays and apodisations, and the adjoint of
-the natural-spline prefilter or of the k-order Hermite coefficients).
+the natural-spline prefilter, of the k-order Hermite coefficients or of the
+Akima / makima coefficients).
 
 The forward is pycbf's Tabbed DAS (pycbf/cpu/pycbf.c:179-271 with the GPU
 layouts of pycbf/gpu/__engines__.cu:813-864); pycbf itself has no gradients.
@@ -15,7 +16,7 @@ import ctypes
 from ..base import BeamformerException
 from . import _engine as E
 
-DIFF_KINDS = ("nearest", "linear", "cubic", "korder_cubic")
+DIFF_KINDS = ("nearest", "linear", "cubic", "korder_cubic", "akima", "makima")
 
 
 def _torch():
@@ -126,8 +127,8 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype
     rf (ntx, nrx, nt) or (nfr, ntx, nrx, nt); tautx / apodtx (ntx, np);
     taurx / apodrx (nrx, np); all float32 CUDA tensors, any of which may
     require grad.  ``interp`` is 'nearest', 'linear', 'cubic' (natural
-    spline) or 'korder_cubic' (``k``, default 8); the output has the layout
-    of TabbedBeamformer for ``sumtype``.
+    spline), 'korder_cubic' (``k``, default 8), 'akima' or 'makima'; the
+    output has the layout of TabbedBeamformer for ``sumtype``.
     ``thresh`` (default 0: every sample) drops samples with
     |apodtx * apodrx| < thresh in the forward and the adjoint alike."""
     global _FN

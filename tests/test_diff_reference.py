@@ -73,7 +73,25 @@ def test_linear_in_channel_data():
                                atol=1e-12)
 
 
-@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic"])
+@pytest.mark.parametrize("method", ["akima", "makima"])
+def test_akima_coefs_match_scipy(method):
+    """The Hermite pieces of diff64.akima_coefs evaluate to scipy's
+    Akima1DInterpolator (same two-point end extension) everywhere, ends
+    included, and pass through the samples."""
+    from scipy.interpolate import Akima1DInterpolator
+    n = 50
+    y = np.random.default_rng(21).standard_normal(n)
+    C = diff64.akima_coefs(torch.as_tensor(y), method == "makima").numpy()
+    xs = np.arange(n - 1)[:, None] + np.array([0.0, 0.2, 0.5, 0.77, 0.999])[None, :]
+    t = xs - np.arange(n - 1)[:, None]
+    c = C[:-1]
+    got = c[:, :1] + t * (c[:, 1:2] + t * (c[:, 2:3] + t * c[:, 3:4]))
+    ref = Akima1DInterpolator(np.arange(float(n)), y, method=method)(xs)
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+    np.testing.assert_allclose(C[:, 0], y, atol=0)
+
+
+@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic", "akima", "makima"])
 def test_autograd_matches_central_differences(kind):
     from oracle import numerics
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(5)

@@ -70,7 +70,7 @@ NAMES = ["rf", "tautx", "apodtx", "taurx", "apodrx"]
 
 
 @pytest.mark.parametrize("sumtype", SUMTYPES)
-@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
+@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic", "akima", "makima"])
 def test_adjoint_matches_fp64_autograd(kind, sumtype):
     print(f"adjoint {kind} {sumtype}", flush=True)
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(11, nfr=2)
@@ -89,7 +89,7 @@ def test_adjoint_matches_fp64_autograd(kind, sumtype):
         assert err <= 1e-4 * np.max(np.abs(b)), f"{kind} {sumtype} d{name}: {err:.3e}"
 
 
-@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic"])
+@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic", "makima"])
 def test_adjoint_threshold_matches_forward_gate(kind):
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(13, discrete=True)
     arrs = [rf, tautx, apodtx, taurx, apodrx]
@@ -162,7 +162,7 @@ def test_cubic_rf_adjoint_trace_lengths(nt):
         assert float(np.abs(g[0] - g1[0]).max()) <= 1e-6 * peak
 
 
-@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
+@pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic", "akima", "makima"])
 def test_rf_adjoint_transpose_identity_full_size(kind):
     """<A rf, G> = <rf, A^T G> for the 31 x 128 x 4096, 256 x 256 workload."""
     import torch
@@ -186,24 +186,29 @@ def test_rf_adjoint_transpose_identity_full_size(kind):
     assert float(grf.abs().max()) > 0
 
 
-@pytest.mark.parametrize("nt,k", [(8, 8), (9, 8), (160, 4), (300, 16), (4097, 8), (4100, 8)])
-def test_korder_adjoint_window_edges_and_long_traces(nt, k):
-    """korder_cubic adjoint against FP64 autograd for windows clamped at both
-    ends (nt close to k) and for traces past one workgroup's LDS bins (the
-    per-tile path with global atomics beyond 4097 samples)."""
+@pytest.mark.parametrize("kind,nt,k", [("korder_cubic", 8, 8), ("korder_cubic", 9, 8),
+                                        ("korder_cubic", 160, 4), ("korder_cubic", 300, 16),
+                                        ("korder_cubic", 4097, 8), ("korder_cubic", 4100, 8),
+                                        ("akima", 5, None), ("makima", 6, None),
+                                        ("makima", 4097, None), ("akima", 4100, None)])
+def test_hermite_adjoint_window_edges_and_long_traces(kind, nt, k):
+    """korder_cubic / akima / makima adjoints against FP64 autograd for
+    windows and end slopes clamped at both ends (short traces) and for traces
+    past one workgroup's LDS bins (the per-tile path with global atomics
+    beyond 4097 samples)."""
     import torch
     from oracle import diff64, numerics
     from pycbf_amd.gpu.autograd import das_tabbed
-    print(f"korder adjoint nt={nt} k={k}", flush=True)
+    print(f"{kind} adjoint nt={nt} k={k}", flush=True)
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(31 + nt, ntx=2, nrx=5, nt=nt, np_=260)
     arrs = [rf, tautx, apodtx, taurx, apodrx]
     gout = np.random.default_rng(32).standard_normal((1, 260)).astype(np.float32)
     t = [torch.as_tensor(a, device="cuda").requires_grad_() for a in arrs]
-    out = das_tabbed(*t, t0, dt, "korder_cubic", "tx_and_rx", 0.0, k=k)
+    out = das_tabbed(*t, t0, dt, kind, "tx_and_rx", 0.0, k=k)
     (out * torch.as_tensor(gout, device="cuda")).sum().backward()
-    S = numerics.make_S(k).astype(np.float32).astype(np.float64)
+    S = numerics.make_S(k).astype(np.float32).astype(np.float64) if k else None
     r = [torch.as_tensor(a, dtype=torch.float64).requires_grad_() for a in arrs]
-    ref = diff64.das_tabbed(*r, t0, dt, "korder_cubic", "tx_and_rx", 0.0, S=S)
+    ref = diff64.das_tabbed(*r, t0, dt, kind, "tx_and_rx", 0.0, S=S)
     (ref * torch.as_tensor(gout, dtype=torch.float64)).sum().backward()
     o, ro = out.detach().cpu().numpy(), ref.detach().numpy()
     # float32 sample positions x <= t0 + nt dt carry a relative weight error of
