@@ -207,10 +207,11 @@ int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int
 
 /* Bytes of workspace pycbf_das_tabbed_bwd needs (cubic: spline table and
  * cotangent bins of every channel; korder_cubic: Hermite table and the
- * cotangents of its four coefficients, as for akima and makima; all kinds:
+ * cotangents of its four coefficients, as for akima and makima; nearest with
+ * param = usf > 1: the upsampled trace and its cotangents; all kinds:
  * per-tile delay ranges). */
 size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
-                                     int interp_kind);
+                                     int interp_kind, int param);
 
 /* Adjoint of pycbf_das_tabbed (no reference counterpart: pycbf is forward
  * only; the forward it differentiates is pycbf/cpu/pycbf.c:179-271 with the
@@ -219,7 +220,8 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
  *   grf      (nfr, ntx, nrx, nt)  d loss / d rf
  *   gtautx, gapodtx (ntx, np); gtaurx, gapodrx (nrx, np)
  * Any output pointer may be NULL (not computed).  interp_kind nearest
- * (param = usf = 1; its delay gradient is 0), linear, cubic (natural spline:
+ * (param = usf; its delay gradient is 0; with usf > 1 the adjoint of the
+ * spline upsampling is applied to grf), linear, cubic (natural spline:
  * the adjoint of the spline prefilter is applied to grf) or korder_cubic
  * (param = k, S the (k, k) operator as for pycbf_das_tabbed; the adjoint of
  * the Hermite coefficient construction is applied to grf), akima or makima

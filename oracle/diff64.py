@@ -12,7 +12,9 @@ delays and apodisations:
   j = floor((x - t0) / dt) clamped to n-1, zero outside [t0, t0 + dt (n-1)]
   a = (t0 + dt (j+1) - x) / dt,  b = (x - (t0 + dt j)) / dt
   linear: v = a y_j + b y_j+1;  cubic: v += dt^2/6 ((a^3-a) y2_j + (b^3-b) y2_j+1)
-  nearest: v = y_k, k = (int)((x - t0)/dt + 0.5), zero unless -1 < (x - t0)/dt + 0.5 < n
+  nearest: v = y_k, k = (int)((x - t0)/dt + 0.5), zero unless -1 < (x - t0)/dt + 0.5 < n;
+    with usf > 1 on the natural spline sampled at dt/usf (pycbf/cpu/pycbf.c:209-271),
+    the fine positions in float32 as the upsampling kernel computes them
   korder_cubic: t = ((x - t0) - dt j) / dt, v = c0 + c1 t + c2 t^2 + c3 t^3 with the
     Hermite coefficients of pycbf/gpu/__engines__.cu:292-355 (end values y_j, y_j+1 and
     derivatives d0, d1 = rows ik0, ik0+1 of S applied to the k-sample window at iy0,
@@ -98,14 +100,42 @@ def akima_coefs(rf: torch.Tensor, modified: bool) -> torch.Tensor:
     return torch.cat([c, last], dim=-2)
 
 
+def upsample(rf: torch.Tensor, t0: float, dt: float, usf: int) -> torch.Tensor:
+    """The natural spline through rf (..., n) at the usf (n-1) + 1 float32 fine
+    positions t0 + it * (dt / usf) (zero past the last knot), in float64."""
+    import numpy as np
+    n = rf.shape[-1]
+    ntu = usf * (n - 1) + 1
+    t0f, dtf = np.float32(t0), np.float32(dt)
+    dtuf = np.float32(dtf / np.float32(usf))
+    xf = t0f + np.arange(ntu, dtype=np.float32) * dtuf
+    ok = ~((xf < t0f) | (xf > t0f + dtf * np.float32(n - 1)))
+    j = np.minimum(((xf - t0f) / dtf).astype(np.int64), n - 1)
+    xd, t0d, dtd = xf.astype(np.float64), float(t0f), float(dtf)
+    a = torch.as_tensor((t0d + dtd * (j + 1) - xd) / dtd)
+    b = torch.as_tensor((xd - (t0d + dtd * j)) / dtd)
+    zpad = torch.zeros(rf.shape[:-1] + (1,), dtype=rf.dtype)
+    y = torch.cat([rf, zpad], -1)
+    y2 = torch.cat([natural_y2(rf, dtd), zpad], -1)
+    jt = torch.as_tensor(j)
+    u = (a * y[..., jt] + b * y[..., jt + 1]
+         + (dtd * dtd / 6.0) * ((a ** 3 - a) * y2[..., jt] + (b ** 3 - b) * y2[..., jt + 1]))
+    return u * torch.as_tensor(ok, dtype=rf.dtype)
+
+
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="tx_and_rx",
-               thresh=0.0, x32=True, S=None):
+               thresh=0.0, x32=True, S=None, usf=1):
     """rf (nfr, ntx, nrx, n); tables (ntx, np) / (nrx, np); returns float64
     with the sumtype layout of the GPU beamformers (frames first).  x32 sums
     the delays in float32 as the kernels do (same sample indices); x32=False
     keeps the whole function in float64 (finite-difference checks).  S: the
-    (k, k) derivative operator of korder_cubic."""
+    (k, k) derivative operator of korder_cubic; usf: the upsampling factor
+    of nearest."""
     rf = rf.to(torch.float64)
+    if kind == "nearest" and usf > 1:
+        # nearest on the upsampled trace at dt / usf (float32, as the kernels)
+        rf = upsample(rf, t0, dt, usf)
+        dt = float(torch.tensor(dt, dtype=torch.float32) / torch.tensor(float(usf), dtype=torch.float32))
     nfr, ntx, nrx, n = rf.shape
     if x32:
         # float32 delay sum, and sample indices / trace bounds from float32

@@ -515,10 +515,11 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 
 // ------------------------------------------------------------- backward ---
 struct BwdPlan {
-    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, gc2 = 0, gc3 = 0, mbits = 256, total = 0;
+    size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, gc2 = 0, gc3 = 0, mbits = 256;
+    size_t y2 = 0, up = 0, gu = 0, total = 0;  // nearest, usf > 1
 };
 
-static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind)
+static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind, int param)
 {
     BwdPlan b;
     const int64_t nt = info->tinfo.nx;
@@ -532,16 +533,23 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
                kind == PYCBF_INTERP_MAKIMA) {
         b.coef = align_up((size_t)nch * (size_t)nt * 16, 256);
         b.gy = b.ge = b.gc2 = b.gc3 = align_up((size_t)nch * (size_t)nt * 4, 256);
+    } else if (kind == PYCBF_INTERP_NEAREST && param > 1) {
+        const size_t ntu = (size_t)param * (size_t)(nt - 1) + 1;
+        b.y2 = align_up((size_t)nch * (size_t)nt * 4, 256);
+        b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
+        b.up = b.gu = align_up((size_t)nch * ntu * 4, 256);
+        b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
     }
-    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.gc2 + b.gc3 + b.mbits + 8 * 256;
+    b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.gc2 + b.gc3 + b.mbits + b.y2 + b.up +
+              b.gu + 11 * 256;
     return b;
 }
 
 size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int64_t np,
-                                     int interp_kind)
+                                     int interp_kind, int param)
 {
     if (!info || nfr < 1 || np < 1) return 0;
-    return plan_bwd(info, nfr, np, interp_kind).total;
+    return plan_bwd(info, nfr, np, interp_kind, param).total;
 }
 
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
@@ -558,16 +566,16 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                     "the adjoint supports nearest, linear, cubic, korder_cubic, akima and makima "
                     "(got %d)",
                     interp_kind);
-    if (interp_kind == PYCBF_INTERP_NEAREST && param != 1)
-        return fail(PYCBF_EINVAL, "the nearest adjoint needs usf = 1");
-    const int kparam = interp_kind == PYCBF_INTERP_KORDER_CUBIC ? param : 1;
+    const int kparam =
+        (interp_kind == PYCBF_INTERP_KORDER_CUBIC || interp_kind == PYCBF_INTERP_NEAREST) ? param
+                                                                                          : 1;
     int rc = validate_common(info, rf, nfr, np, interp_kind, kparam,
                              interp_kind == PYCBF_INTERP_KORDER_CUBIC ? S : nullptr, sumflag, gout);
     if (rc) return rc;
     if (!tautx || !apodtx || !taurx || !apodrx)
         return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
     if (nfr > 65535 || info->ntx > 65535) return fail(PYCBF_EINVAL, "nfr and ntx must be <= 65535");
-    const BwdPlan plan = plan_bwd(info, nfr, np, interp_kind);
+    const BwdPlan plan = plan_bwd(info, nfr, np, interp_kind, kparam);
     if (!workspace || workspace_bytes < plan.total)
         return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
                     workspace_bytes);
@@ -628,6 +636,21 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         b.modified = interp_kind == PYCBF_INTERP_MAKIMA;
         if ((rc = launch_akima_coefs(rf, nch, nt, b.modified, coef, s))) return rc;
         b.src = (const float *)coef;
+    } else if (interp_kind == PYCBF_INTERP_NEAREST && param > 1) {
+        // nearest on the upsampled trace, as the forward gathers it
+        float *y2 = (float *)cv.take(plan.y2);
+        float *u = (float *)cv.take(plan.uscr);
+        float *up = (float *)cv.take(plan.up);
+        if ((rc = launch_spline_y2(rf, nch, nt, dt, u, y2, s))) return rc;
+        if ((rc = launch_upsample(rf, y2, nch, nt, t0, dt, param, up, s))) return rc;
+        const int ntu = param * (nt - 1) + 1;
+        b.src = up;
+        b.nt = ntu;
+        b.dt = dt / (float)param;
+        b.rdt = 1.0f / b.dt;
+        b.tmax = t0 + b.dt * (float)(ntu - 1);
+        b.gy = (float *)cv.take(plan.gu);
+        b.gstride = ntu;
     } else {
         b.gy = grf;
         b.gstride = nt;
@@ -650,7 +673,13 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                    : interp_kind == PYCBF_INTERP_CUBIC  ? GK_CUBIC
                    : interp_kind == PYCBF_INTERP_KORDER_CUBIC ? GK_KORDER
                                                               : GK_AKIMA;
-    return launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    if ((rc = launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s))) return rc;
+    if (grf && interp_kind == PYCBF_INTERP_NEAREST && param > 1) {
+        float *gy = (float *)cv.take(plan.gy);
+        float *ge = (float *)cv.take(plan.ge);
+        return launch_upsample_adjoint(b.gy, nch, nt, param, t0, dt, gy, ge, grf, s);
+    }
+    return 0;
 }
 
 // ------------------------------------------------------- speed of sound ---

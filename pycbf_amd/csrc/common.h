@@ -139,6 +139,10 @@ struct BwdArgs {
 };
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
                    float *gapodrx, float *grf, hipStream_t s);
+// nearest with usf > 1: cotangents gu of the upsampled trace (nch, usf (nt-1)
+// + 1) -> spline-table cotangents gy, ge (nch, nt + 1) -> grf (nch, nt)
+int launch_upsample_adjoint(const float *gu, int64_t nch, int32_t nt, int32_t usf, float t0,
+                            float dt, float *gy, float *ge, float *grf, hipStream_t s);
 
 // ------------------------------------------------------------ delay tabs ---
 int launch_tabs_2d(int mode, int ntx, const double *txc, int nrx, const double *xele,

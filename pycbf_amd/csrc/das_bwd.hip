@@ -1016,7 +1016,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
                 if (int rc = fill_zero(a.gc3, nch * a.gstride, s)) return rc;
             }
             if (GK == GK_NEAREST || GK == GK_LINEAR) {
-                if (int rc = fill_zero(grf, nch * a.nt, s)) return rc;
+                // the cotangent rows (grf itself, or the upsampled grid's rows)
+                if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
             }
             hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
         }
@@ -1051,6 +1052,74 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         }
     }
     return 0;
+}
+
+// Adjoint of k_upsample (natural-spline values at usf x the sampling rate,
+// pycbf/cpu/pycbf.c:209-271 with usf > 1): the cotangents gu of the upsampled
+// trace (nch, usf (nt-1) + 1) go back to the cotangents of the spline table
+// entries y_i and e_i = dt^2/6 y2_i (rows of nt + 1, entry nt unused).  Fine
+// point it selects interval j with the forward's float32 operations and
+// contributes a gu, (a^3 - a) gu to entry j and b gu, (b^3 - b) gu to entry
+// j+1 (when j+1 < nt).  One lane per (channel, entry) gathers over the fine
+// points that can select it (2 usf + 5 candidates); no atomics.
+__global__ __launch_bounds__(256) void k_upsample_adjoint(const float *__restrict__ gu,
+                                                          int64_t nch, int nt, int usf, float t0,
+                                                          float dt, float *__restrict__ gy,
+                                                          float *__restrict__ ge)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int gs = nt + 1;
+    if (g >= nch * gs) return;
+    const int64_t ch = g / gs;
+    const int i = (int)(g - ch * gs);
+    const int ntu = usf * (nt - 1) + 1;
+    const float dtu = dt / (float)usf;
+    const float xmax = t0 + dt * (float)(nt - 1);
+    const float *row = gu + ch * (int64_t)ntu;
+    float sy = 0.0f, se = 0.0f;
+    const int lo = max(0, usf * (i - 1) - 2), hi = min(ntu - 1, usf * (i + 1) + 2);
+    for (int it = lo; it <= hi; ++it) {
+        const float x = t0 + (float)it * dtu;
+        if ((x < t0) || (x > xmax)) continue;
+        const int j = min((int)((x - t0) / dt), nt - 1);
+        if (j != i && j + 1 != i) continue;
+        const float xlead = t0 + dt * (float)(j + 1);
+        const float xlag = t0 + dt * (float)j;
+        const float v = row[it];
+        if (j == i) {
+            const float a = (xlead - x) / dt;
+            sy = __builtin_fmaf(a, v, sy);
+            se = __builtin_fmaf(__builtin_fmaf(a * a, a, -a), v, se);
+        } else if (j + 1 < nt) {
+            const float b = (x - xlag) / dt;
+            sy = __builtin_fmaf(b, v, sy);
+            se = __builtin_fmaf(__builtin_fmaf(b * b, b, -b), v, se);
+        }
+    }
+    gy[g] = sy;
+    ge[g] = se;
+}
+
+int launch_upsample_adjoint(const float *gu, int64_t nch, int32_t nt, int32_t usf, float t0,
+                            float dt, float *gy, float *ge, float *grf, hipStream_t s)
+{
+    if (nch <= 0) return 0;
+    if (nt > SPL_MAX_NT) return fail(PYCBF_EINVAL, "spline adjoint: nt too large");
+    const int64_t n = nch * ((int64_t)nt + 1);
+    hipLaunchKernelGGL(k_upsample_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       gu, nch, (int)nt, (int)usf, t0, dt, gy, ge);
+    if (int rc = check_launch("upsample_adjoint")) return rc;
+    const int seg = spline_seg_len(nt);
+    if (seg == 32)
+        hipLaunchKernelGGL(k_spline_adjoint_seg<32>, dim3((unsigned)nch), dim3(64), 0, s, gy, ge,
+                           nt + 1, (int)nt, grf);
+    else if (seg == 64)
+        hipLaunchKernelGGL(k_spline_adjoint_seg<64>, dim3((unsigned)nch), dim3(64), 0, s, gy, ge,
+                           nt + 1, (int)nt, grf);
+    else
+        hipLaunchKernelGGL(k_spline_adjoint, dim3((unsigned)((nch + 63) / 64)), dim3(64), 0, s,
+                           gy, ge, nt + 1, nch, (int)nt, grf);
+    return check_launch("spline_adjoint");
 }
 
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,

@@ -98,7 +98,7 @@ _SIGS = {
                     _vp, ct.c_int, ct.c_int, _vp, _vp], ct.c_int),
     "pycbf_slsc_selection": ([_vp, _i64, _i64, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                               _vp, _vp], ct.c_int),
-    "pycbf_das_bwd_workspace_bytes": ([_prf, _i64, _i64, ct.c_int], _sz),
+    "pycbf_das_bwd_workspace_bytes": ([_prf, _i64, _i64, ct.c_int, ct.c_int], _sz),
     "pycbf_das_tabbed_bwd": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _i64, ct.c_int, ct.c_int,
                               _vp, _f32, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp],
                              ct.c_int),

@@ -42,7 +42,7 @@ def _out_shape(sumtype, nfr, ntx, nrx, np_):
 class _TabbedDAS:
     """Container of the non-tensor arguments (hashable-free, kept on ctx)."""
 
-    def __init__(self, t0, dt, kind, sumtype, thresh, k=None):
+    def __init__(self, t0, dt, kind, sumtype, thresh, k=None, usf=1):
         if kind not in DIFF_KINDS:
             raise BeamformerException(f"differentiable kinds are {DIFF_KINDS}, got {kind!r}")
         if sumtype not in E.SUMTYPE:
@@ -50,6 +50,12 @@ class _TabbedDAS:
         self.t0, self.dt, self.kind, self.sumtype = float(t0), float(dt), kind, sumtype
         self.thresh = float(thresh)
         self.param, self.S = 1, None
+        if kind == "nearest":
+            if int(usf) != usf or usf < 1:
+                raise BeamformerException("usf must be an integer >= 1")
+            self.param = int(usf)
+        elif usf != 1:
+            raise BeamformerException("usf applies to the nearest kind only")
         if kind == "korder_cubic":
             from .core import make_S_by_k
             k = 8 if k is None else int(k)
@@ -60,7 +66,7 @@ class _TabbedDAS:
 
     def S_on(self, device):
         """The (k, k) operator on ``device`` (korder_cubic), else None."""
-        if self.S is None and self.param > 1:
+        if self.S is None and self.kind == "korder_cubic":
             self.S = _torch().as_tensor(self.S_host, device=device)
         return self.S
 
@@ -75,7 +81,7 @@ def _das_function():
             np_ = tautx.shape[1]
             info = E.rfinfo(ntx, nrx, 0, cfg.t0, cfg.dt, nt)
             kind = E.INTERP[cfg.kind]
-            need = E.lib.pycbf_das_workspace_bytes(ctypes.byref(info), nfr, np_, kind, 1, 0)
+            need = E.lib.pycbf_das_workspace_bytes(ctypes.byref(info), nfr, np_, kind, cfg.param, 0)
             ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
             out = torch.empty(_out_shape(cfg.sumtype, nfr, ntx, nrx, np_), dtype=torch.float32,
                               device=rf.device)
@@ -103,7 +109,8 @@ def _das_function():
             gtrx, garx = mk(need_trx, taurx), mk(need_arx, apodrx)
             info = E.rfinfo(ntx, nrx, 0, cfg.t0, cfg.dt, nt)
             kind = E.INTERP[cfg.kind]
-            need = E.lib.pycbf_das_bwd_workspace_bytes(ctypes.byref(info), nfr, np_, kind)
+            need = E.lib.pycbf_das_bwd_workspace_bytes(ctypes.byref(info), nfr, np_, kind,
+                                                       cfg.param)
             ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
             rc = E.lib.pycbf_das_tabbed_bwd(
                 ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx), E.ptr(apodtx), E.ptr(taurx),
@@ -121,20 +128,21 @@ _FN = None
 
 
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype="tx_and_rx",
-               thresh=0.0, k=None):
+               thresh=0.0, k=None, usf=1):
     """Differentiable Tabbed DAS on the GPU.
 
     rf (ntx, nrx, nt) or (nfr, ntx, nrx, nt); tautx / apodtx (ntx, np);
     taurx / apodrx (nrx, np); all float32 CUDA tensors, any of which may
     require grad.  ``interp`` is 'nearest', 'linear', 'cubic' (natural
-    spline), 'korder_cubic' (``k``, default 8), 'akima' or 'makima'; the
+    spline), 'korder_cubic' (``k``, default 8), 'akima' or 'makima';
+    'nearest' takes ``usf`` (the spline upsampling factor, default 1).  The
     output has the layout of TabbedBeamformer for ``sumtype``.
     ``thresh`` (default 0: every sample) drops samples with
     |apodtx * apodrx| < thresh in the forward and the adjoint alike."""
     global _FN
     if _FN is None:
         _FN = _das_function()
-    cfg = _TabbedDAS(t0, dt, interp, sumtype, thresh, k)
+    cfg = _TabbedDAS(t0, dt, interp, sumtype, thresh, k, usf)
     squeeze = rf.dim() == 3
     rf4 = rf[None] if squeeze else rf
     if rf4.dim() != 4:

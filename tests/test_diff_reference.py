@@ -63,6 +63,27 @@ def test_korder_coefs_reproduce_samples_and_cubics():
             assert abs(float(v) - poly(j + t)) <= 1e-9 * (1 + abs(poly(j + t)))
 
 
+@pytest.mark.parametrize("usf", [2, 3, 4])
+def test_nearest_upsampled_agrees_with_float32_oracle(usf):
+    rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(8)
+    f32 = [np.float32(a) for a in (rf, tautx, apodtx, taurx, apodrx)]
+    got = diff64.das_tabbed(*(torch.as_tensor(a) for a in f32), t0, dt, "nearest",
+                            usf=usf).numpy()[0]
+    ref = oracle.das_tabbed(f32[0][0], t0, dt, *f32[1:], kind="nearest", usf=usf, thresh=0.0)
+    # the oracle upsamples with its float32 spline recurrence (about 2e-5 of
+    # the peak here); a wrong weight or index would be of the order of the peak
+    assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+def test_upsample_matches_scipy_natural_spline():
+    from scipy.interpolate import CubicSpline
+    y = np.random.default_rng(10).standard_normal((1, 40))
+    u = diff64.upsample(torch.as_tensor(y), 0.0, 0.25, 4).numpy()[0]
+    xs = np.float32(0.0) + np.arange(4 * 39 + 1, dtype=np.float32) * np.float32(np.float32(0.25) / 4)
+    ref = CubicSpline(np.arange(40) * 0.25, y[0], bc_type="natural")(xs.astype(np.float64))
+    np.testing.assert_allclose(u, ref, atol=1e-12)
+
+
 def test_linear_in_channel_data():
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(3)
     T = [torch.as_tensor(a) for a in (tautx, apodtx, taurx, apodrx)]
@@ -91,20 +112,25 @@ def test_akima_coefs_match_scipy(method):
     np.testing.assert_allclose(C[:, 0], y, atol=0)
 
 
-@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic", "akima", "makima"])
+@pytest.mark.parametrize("kind", ["linear", "cubic", "korder_cubic", "akima", "makima",
+                                  "nearest_usf3"])
 def test_autograd_matches_central_differences(kind):
     from oracle import numerics
     rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(5)
     S = numerics.make_S(8) if kind == "korder_cubic" else None
+    usf = 3 if kind == "nearest_usf3" else 1
+    kind = "nearest" if kind == "nearest_usf3" else kind
     args = [torch.as_tensor(a, dtype=torch.float64).requires_grad_()
             for a in (rf, tautx, apodtx, taurx, apodrx)]
     gout = torch.as_tensor(np.random.default_rng(6).standard_normal((1, 40)))
-    loss = (diff64.das_tabbed(*args, t0, dt, kind, x32=False, S=S) * gout).sum()
-    grads = torch.autograd.grad(loss, args)
+    loss = (diff64.das_tabbed(*args, t0, dt, kind, x32=False, S=S, usf=usf) * gout).sum()
+    grads = [torch.zeros_like(a) if g is None else g
+             for a, g in zip(args, torch.autograd.grad(loss, args, allow_unused=True))]
 
     def L(vals):
         with torch.no_grad():
-            return float((diff64.das_tabbed(*vals, t0, dt, kind, x32=False, S=S) * gout).sum())
+            return float((diff64.das_tabbed(*vals, t0, dt, kind, x32=False, S=S, usf=usf)
+                          * gout).sum())
 
     rng = np.random.default_rng(7)
     for ai, h in ((0, 1e-4), (1, 1e-13), (2, 1e-6), (3, 1e-13), (4, 1e-6)):

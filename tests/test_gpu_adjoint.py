@@ -218,3 +218,61 @@ def test_hermite_adjoint_window_edges_and_long_traces(kind, nt, k):
     for name, a, b in zip(NAMES, t, r):
         ga, gb = a.grad.cpu().numpy(), b.grad.numpy()
         assert np.max(np.abs(ga - gb)) <= tol * np.max(np.abs(gb)), name
+
+
+@pytest.mark.parametrize("usf,nt,sumtype", [(2, 160, "tx_and_rx"), (3, 97, "none"),
+                                            (4, 160, "rx_only"), (2, 2100, "tx_and_rx"),
+                                            (4, 1100, "tx_only")])
+def test_nearest_upsampled_adjoint_matches_fp64(usf, nt, sumtype):
+    """nearest with usf > 1: the adjoint on the upsampled grid, then the
+    transposed spline upsampling and the transposed spline solve, against
+    FP64 autograd through the float64 natural spline.  Short traces take the
+    per-channel bins (usf (nt-1) + 1 <= 4097), long ones the per-tile path."""
+    import torch
+    from oracle import diff64
+    from pycbf_amd.gpu.autograd import das_tabbed
+    print(f"nearest usf={usf} nt={nt} {sumtype}", flush=True)
+    rf, tautx, apodtx, taurx, apodrx, t0, dt = _case(40 + usf, ntx=3, nrx=4, nt=nt, np_=300)
+    arrs = [rf, tautx, apodtx, taurx, apodrx]
+    r = [torch.as_tensor(a, dtype=torch.float64).requires_grad_() for a in arrs]
+    ref = diff64.das_tabbed(*r, t0, dt, "nearest", sumtype, 0.0, usf=usf)
+    gout = np.random.default_rng(41).standard_normal(tuple(ref.shape)).astype(np.float32)
+    (ref * torch.as_tensor(gout, dtype=torch.float64)).sum().backward()
+    t = [torch.as_tensor(a, device="cuda").requires_grad_() for a in arrs]
+    out = das_tabbed(*t, t0, dt, "nearest", sumtype, 0.0, usf=usf)
+    (out * torch.as_tensor(gout, device="cuda")).sum().backward()
+    o, ro = out.detach().cpu().numpy(), ref.detach().numpy()
+    # the upsampling weights come from float32 knot times t0 + dt j: a relative
+    # error of 2^-23 (t0/dt + nt) that the float64 reference does not have
+    tol = max(1e-4, 2.0 ** -23 * (t0 / dt + nt))
+    assert np.max(np.abs(o - ro)) <= tol * np.max(np.abs(ro))
+    for name, a, b in zip(NAMES, t, r):
+        ga = a.grad.cpu().numpy()
+        if name.startswith("tau"):
+            assert np.all(ga == 0.0)
+            continue
+        gb = b.grad.numpy()
+        assert np.max(np.abs(ga - gb)) <= tol * np.max(np.abs(gb)), name
+
+
+def test_nearest_upsampled_transpose_identity_full_size():
+    """<A rf, G> = <rf, A^T G> for nearest with usf = 2 at 31 x 128 x 4096,
+    256 x 256 (8191 upsampled samples per channel: the per-tile path)."""
+    import torch
+    from pycbf_amd import sim
+    from pycbf_amd.gpu.autograd import das_tabbed
+    print("transpose identity nearest usf=2", flush=True)
+    xele = sim.linear_array(128)
+    steers = np.radians(np.linspace(-15, 15, 31))
+    _, _, pnts = sim.grid(256, 256, (-19e-3, 19e-3), (5e-3, 45e-3))
+    tabs = [torch.as_tensor(a, device="cuda") for a in sim.pw_tabs(steers, xele, pnts)]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(19)
+    rf = torch.randn((31, 128, 4096), generator=g, device="cuda", requires_grad=True)
+    G = torch.randn((256 * 256,), generator=g, device="cuda")
+    out = das_tabbed(rf, *tabs, 0.0, 1.0 / 40e6, "nearest", "tx_and_rx", 0.0, usf=2)
+    (grf,) = torch.autograd.grad((out * G).sum(), [rf])
+    lhs = float((out.detach().double() * G.double()).sum())
+    rhs = float((rf.detach().double() * grf.double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1e-30), (lhs, rhs)
+    assert float(grf.abs().max()) > 0
