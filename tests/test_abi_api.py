@@ -114,6 +114,46 @@ def test_host_entry_points_report_argument_errors():
     assert E.lib.pycbf_demod_apply(None, 0, 16, None, 8, None, None) == 0
 
 
+def test_adjoint_entry_point_validates_before_device_work():
+    """pycbf_das_tabbed_bwd rejects unsupported kinds, bad k-order arguments,
+    oversized upsampling and short workspaces with a message, before any
+    device call (host buffers stand in for device pointers: never read)."""
+    from pycbf_amd.gpu import _engine as E
+    info = E.rfinfo(2, 3, 0, 0.0, 2.5e-8, 64)
+    buf = np.zeros(4096, np.float32)
+    p = E.ptr(buf)
+
+    def call(kind, param=1, S=None, ws=0):
+        return E.lib.pycbf_das_tabbed_bwd(ct.byref(info), p, 1, p, p, p, p, 16, kind, param,
+                                          S, 0.0, E.SUMTYPE["tx_and_rx"], p, p, None, None,
+                                          None, None, p if ws else None, ws, None)
+
+    assert call(99) != 0
+    assert b"adjoint supports" in E.lib.pycbf_last_error()
+    assert call(E.INTERP["korder_cubic"], 5, p) != 0
+    assert b"even" in E.lib.pycbf_last_error()
+    assert call(E.INTERP["korder_cubic"], 8, None) != 0
+    assert b"S operator" in E.lib.pycbf_last_error()
+    assert call(E.INTERP["nearest"], 1 << 28) != 0
+    assert b"int32" in E.lib.pycbf_last_error()
+    for kind, param in (("cubic", 1), ("korder_cubic", 8), ("akima", 1), ("nearest", 4)):
+        assert call(E.INTERP[kind], param, p if kind == "korder_cubic" else None, ws=64) != 0
+        assert b"workspace" in E.lib.pycbf_last_error()
+
+
+def test_adjoint_workspace_grows_with_kind_and_upsampling():
+    from pycbf_amd.gpu import _engine as E
+    info = E.rfinfo(5, 16, 0, 0.0, 2.5e-8, 1024)
+    q = lambda kind, param=1: E.lib.pycbf_das_bwd_workspace_bytes(  # noqa: E731
+        ct.byref(info), 2, 4096, E.INTERP[kind], param)
+    nearest, up4 = q("nearest"), q("nearest", 4)
+    # the upsampled trace and its cotangents: 2 x nch x (4 (nt-1) + 1) floats
+    assert up4 - nearest >= 2 * 2 * 5 * 16 * (4 * 1023 + 1) * 4
+    # Hermite kinds: a float4 table and four cotangent rows per channel
+    assert q("korder_cubic", 8) >= nearest + 2 * 5 * 16 * 1024 * (16 + 4 * 4)
+    assert q("akima") == q("makima") == q("korder_cubic", 8)
+
+
 def test_workspace_query_is_host_only_and_grows_with_frames():
     from pycbf_amd.gpu import _engine as E
     info = E.rfinfo(5, 128, 0, 0.0, 2.5e-8, 2048)
