@@ -156,3 +156,25 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype
     apodrx = _check(apodrx, "apodrx", (nrx, np_))
     out = _FN.apply(rf4, tautx, apodtx, taurx, apodrx, cfg)
     return out[0] if squeeze else out
+
+
+def das_beamformer(rf, bf, tables=None):
+    """Differentiable forward of a TabbedBeamformer or SyntheticBeamformer
+    ``bf`` with respect to the channel data (and, for tables passed in, the
+    tables): its interpolation, sumtype and thresh, through das_tabbed.  A
+    SyntheticBeamformer's geometry is evaluated once into device tables
+    (``bf.tables()``: pycbf_synthetic_tables, the float32 operations its
+    gather evaluates per sample), so the geometry itself is a constant.
+    ``tables`` (tautx, apodtx, taurx, apodrx) overrides the beamformer's."""
+    torch = _torch()
+    if tables is None:
+        if hasattr(bf, "tables"):
+            tables = bf.tables()
+        else:
+            tables = tuple(torch.as_tensor(getattr(bf, n), dtype=torch.float32,
+                                           device=rf.device).contiguous()
+                           for n in ("tautx", "apodtx", "taurx", "apodrx"))
+    kind = bf.interp["kind"]
+    return das_tabbed(rf, *tables, bf.t0, bf.dt, kind, bf.sumtype, bf.thresh,
+                      k=bf.interp.get("k") if kind == "korder_cubic" else None,
+                      usf=int(bf.interp.get("usf", 1)) if kind == "nearest" else 1)

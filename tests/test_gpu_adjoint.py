@@ -276,3 +276,33 @@ def test_nearest_upsampled_transpose_identity_full_size():
     rhs = float((rf.detach().double() * grf.double()).sum())
     assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1e-30), (lhs, rhs)
     assert float(grf.abs().max()) > 0
+
+
+@pytest.mark.parametrize("mode,interp", [("pw", {"kind": "cubic"}), ("foc", {"kind": "linear"}),
+                                         ("div", {"kind": "korder_cubic", "k": 8}),
+                                         ("dof", {"kind": "nearest", "usf": 2})])
+def test_das_beamformer_differentiates_synthetic_geometry_forward(mode, interp):
+    """das_beamformer on a SyntheticBeamformer: the forward equals the
+    beamformer's own (in-register geometry) image, and the channel-data
+    gradient satisfies <A rf, G> = <rf, A^T G>."""
+    import torch
+    from pycbf_amd.gpu import SyntheticBeamformer
+    from pycbf_amd.gpu.autograd import das_beamformer
+    from tests.helpers import small_synthetic
+    print(f"das_beamformer {mode} {interp}", flush=True)
+    w = small_synthetic(seed=3, mode=mode)
+    bf = SyntheticBeamformer(**w["params"], t0=w["t0"], dt=w["dt"], nt=w["nt"], interp=interp,
+                             sumtype="tx_and_rx")
+    ref = np.asarray(bf(w["rf"]))
+    rf = torch.as_tensor(w["rf"].astype(np.float32), device="cuda").requires_grad_()
+    out = das_beamformer(rf, bf)
+    got = out.detach().cpu().numpy().reshape(ref.shape)
+    peak = float(np.abs(ref).max())
+    assert peak > 0
+    assert float(np.abs(got - ref).max()) <= 1e-6 * peak
+    G = torch.as_tensor(np.random.default_rng(5).standard_normal(tuple(out.shape)),
+                        dtype=torch.float32, device="cuda")
+    (grf,) = torch.autograd.grad((out * G).sum(), [rf])
+    lhs = float((out.detach().double() * G.double()).sum())
+    rhs = float((rf.detach().double() * grf.double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + 1e-30), (lhs, rhs)
