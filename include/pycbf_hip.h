@@ -168,7 +168,13 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
  * output sample m, so iq reads as complex64 (rows, ncol/2).  All pointers
  * are device pointers, rows row-major and dense; ncol <= 128 * 65535. */
 int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *opT, int64_t ncol,
-                      float *iq, void *stream);
+                      float *iq, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Bytes of device workspace pycbf_demod_apply needs for rows lines of n
+ * samples and ncol outputs: 0 when the output tiles fill the GPU, else the
+ * partial products of a split over the samples (few lines, long n).  No
+ * reference counterpart (the notebooks' FFT chain needs none). */
+size_t pycbf_demod_workspace_bytes(int64_t rows, int64_t n, int64_t ncol);
 
 /* HOST function.  Spatially compact processing order for np pixel positions
  * (host pvec (np, ndim)) written to host perm (np).  No reference counterpart:

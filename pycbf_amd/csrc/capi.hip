@@ -341,15 +341,27 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
                           taurx, apodrx, (hipStream_t)stream);
 }
 
+size_t pycbf_demod_workspace_bytes(int64_t rows, int64_t n, int64_t ncol)
+{
+    if (rows < 1 || n < 1 || ncol < 1 || n > 0x7fffffffLL || ncol > 128LL * 65535) return 0;
+    const int nsplit = demod_split(rows, (int)n, (int)ncol);
+    return nsplit > 1 ? (size_t)nsplit * (size_t)rows * (size_t)ncol * sizeof(float) : 0;
+}
+
 int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *opT, int64_t ncol,
-                      float *iq, void *stream)
+                      float *iq, void *workspace, size_t workspace_bytes, void *stream)
 {
     if (rows < 0 || n < 1 || ncol < 1) return fail(PYCBF_EINVAL, "need rows >= 0, n >= 1, ncol >= 1");
     if (n > 0x7fffffffLL) return fail(PYCBF_EINVAL, "n must fit in int32");
     if (ncol > 128LL * 65535) return fail(PYCBF_EINVAL, "ncol must be <= %d", 128 * 65535);
     if (rows > 0 && (!rf || !opT || !iq)) return fail(PYCBF_EINVAL, "NULL device pointer");
     if ((rows + 127) / 128 > 0x7fffffffLL) return fail(PYCBF_EINVAL, "too many rows");
-    return launch_demod_gemm(rf, rows, (int)n, opT, (int)ncol, iq, (hipStream_t)stream);
+    const size_t need = pycbf_demod_workspace_bytes(rows, n, ncol);
+    if (need > 0 && (!workspace || workspace_bytes < need))
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes);
+    return launch_demod_gemm(rf, rows, (int)n, opT, (int)ncol, iq,
+                             need > 0 ? (float *)workspace : nullptr, (hipStream_t)stream);
 }
 
 int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,

@@ -152,9 +152,11 @@ int launch_tabs_2d(int mode, int ntx, const double *txc, int nrx, const double *
 
 // ------------------------------------------------------------ demodulation ---
 // C (R x N) = A (R x K) @ Bt (N x K)^T, row-major float32, float32-accurate
-// (three-way BF16 split on the BF16 matrix units)
+// (three-way BF16 split on the BF16 matrix units); part: demod_split(R, K, N)
+// x R x N floats of partial products when that is above 1
+int demod_split(int64_t R, int K, int N);
 int launch_demod_gemm(const float *A, int64_t R, int K, const float *Bt, int N, float *C,
-                      hipStream_t s);
+                      float *part, hipStream_t s);
 
 // ------------------------------------------------------------------ DMAS ---
 int launch_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,

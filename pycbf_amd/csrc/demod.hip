@@ -23,6 +23,8 @@
 // the workgroup loads the next slice's rf and opT tiles into registers while
 // the current one (three BF16 planes each, in LDS) feeds the MFMAs; edges are
 // zero-filled.
+#include <algorithm>
+
 #include "common.h"
 #include "prof.h"
 #include "tiles.h"
@@ -37,10 +39,10 @@ constexpr int DM = 128;     // workgroup tile rows (lines) and columns
 constexpr int DK = 32;      // k slice
 constexpr int DP = DK + 8;  // LDS row pitch in BF16 (80 bytes: rows stay 16-byte aligned)
 
-// rows [row0, row0 + 128) x k [k0, k0 + 32) of a (nrow x K) float32 matrix:
-// thread t holds rows t/8 + 32q (q < 4), k 4(t%8) .. +3
-__device__ __forceinline__ void load_tile(const float *__restrict__ M, int64_t nrow, int K,
-                                          int64_t row0, int k0, F4u (&v)[4])
+// rows [row0, row0 + 128) x k [k0, k0 + 32) of a (nrow x ld) float32 matrix,
+// zero from column kend on: thread t holds rows t/8 + 32q (q < 4), k 4(t%8) .. +3
+__device__ __forceinline__ void load_tile(const float *__restrict__ M, int64_t nrow, int ld,
+                                          int kend, int64_t row0, int k0, F4u (&v)[4])
 {
     const int t = threadIdx.x;
     const int k = k0 + 4 * (t & 7);
@@ -49,13 +51,13 @@ __device__ __forceinline__ void load_tile(const float *__restrict__ M, int64_t n
         const int64_t r = row0 + (t >> 3) + 32 * q;
         v[q] = F4u{{0.0f, 0.0f, 0.0f, 0.0f}};
         if (r < nrow) {
-            const float *src = M + r * (int64_t)K + k;
-            if (k + 4 <= K) {
+            const float *src = M + r * (int64_t)ld + k;
+            if (k + 4 <= kend) {
                 v[q] = *reinterpret_cast<const F4u *>(src);
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    if (k + e < K) v[q].v[e] = src[e];
+                    if (k + e < kend) v[q].v[e] = src[e];
             }
         }
     }
@@ -88,8 +90,10 @@ __device__ __forceinline__ void store_split(__bf16 *__restrict__ s, const F4u (&
 }
 }  // namespace
 
+// blockIdx.z: the slice [z kc, (z+1) kc) of k; with several slices C is the
+// (nsplit, R, N) array of partial products
 __global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A, int64_t R, int K,
-                                                    const float *__restrict__ Bt, int N,
+                                                    const float *__restrict__ Bt, int N, int kc,
                                                     float *__restrict__ C)
 {
     __shared__ __align__(16) __bf16 As[3 * DM * DP];
@@ -106,17 +110,18 @@ __global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A,
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x16_t{};
 
+    const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
     F4u va[4], vb[4];
-    load_tile(A, R, K, row0, 0, va);
-    load_tile(Bt, N, K, col0, 0, vb);
-    for (int k0 = 0; k0 < K; k0 += DK) {
+    load_tile(A, R, K, kend, row0, kbeg, va);
+    load_tile(Bt, N, K, kend, col0, kbeg, vb);
+    for (int k0 = kbeg; k0 < kend; k0 += DK) {
         store_split(As, va);
         store_split(Bs, vb);
         __syncthreads();
-        if (k0 + DK < K) {
+        if (k0 + DK < kend) {
             // next slice in flight while this one multiplies
-            load_tile(A, R, K, row0, k0 + DK, va);
-            load_tile(Bt, N, K, col0, k0 + DK, vb);
+            load_tile(A, R, K, kend, row0, k0 + DK, va);
+            load_tile(Bt, N, K, kend, col0, k0 + DK, vb);
         }
 #pragma unroll
         for (int kk = 0; kk < DK; kk += 16) {
@@ -150,6 +155,7 @@ __global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A,
     }
     // D[i][j] of a 32 x 32 block sits in lane j + 32h, register r, with
     // i = 8(r/4) + 4h + r%4
+    C += (int64_t)blockIdx.z * R * N;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -163,15 +169,51 @@ __global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A,
             }
 }
 
+// C[g] = sum over z of P[z][g], z in order (deterministic)
+__global__ __launch_bounds__(256) void k_demod_reduce(const float *__restrict__ P, int nsplit,
+                                                      int64_t n, float *__restrict__ C)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= n) return;
+    float v = P[g];
+    for (int z = 1; z < nsplit; ++z) v += P[(int64_t)z * n + g];
+    C[g] = v;
+}
+
+int demod_split(int64_t R, int K, int N)
+{
+    // split k while the tiles alone leave the chip short of about 512
+    // workgroups, keeping at least 256 k (8 slices) per split
+    const int64_t tiles = ((R + DM - 1) / DM) * ((N + DM - 1) / DM);
+    if (tiles >= 512 || K < 512) return 1;
+    const int64_t want = (512 + tiles - 1) / tiles;
+    return (int)std::max<int64_t>(1, std::min<int64_t>({want, (int64_t)K / 256, 64}));
+}
+
 int launch_demod_gemm(const float *A, int64_t R, int K, const float *Bt, int N, float *C,
-                      hipStream_t s)
+                      float *part, hipStream_t s)
 {
     if (R <= 0 || N <= 0) return 0;
-    // row tiles on x (up to 2^31 - 1), column tiles on y
-    dim3 grid((unsigned)((R + DM - 1) / DM), (unsigned)((N + DM - 1) / DM));
+    int nsplit = demod_split(R, K, N);
+    int kc = K;
+    if (nsplit > 1) {
+        kc = ((K + nsplit - 1) / nsplit + DK - 1) / DK * DK;
+        nsplit = (K + kc - 1) / kc;
+    }
+    if (nsplit > 1 && !part) return fail(PYCBF_EINVAL, "demodulation workspace missing");
+    // row tiles on x (up to 2^31 - 1), column tiles on y, k slices on z
+    dim3 grid((unsigned)((R + DM - 1) / DM), (unsigned)((N + DM - 1) / DM), (unsigned)nsplit);
     ProfScope ps("demod_gemm", s);
-    hipLaunchKernelGGL(k_demod_gemm, grid, dim3(256), 0, s, A, R, K, Bt, N, C);
-    return check_launch("demod gemm");
+    hipLaunchKernelGGL(k_demod_gemm, grid, dim3(256), 0, s, A, R, K, Bt, N, kc,
+                       nsplit > 1 ? part : C);
+    if (int rc = check_launch("demod gemm")) return rc;
+    if (nsplit > 1) {
+        const int64_t n = R * (int64_t)N;
+        hipLaunchKernelGGL(k_demod_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           part, nsplit, n, C);
+        return check_launch("demod reduce");
+    }
+    return 0;
 }
 
 }  // namespace pycbf

@@ -87,7 +87,8 @@ _SIGS = {
     "pycbf_tile_order": ([_vp, _i64, ct.c_int, _vp], ct.c_int),
     "pycbf_make_tabs_2d": ([ct.c_int, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, ct.c_double,
                             ct.c_double, _vp, _vp, _vp, _vp, _vp], ct.c_int),
-    "pycbf_demod_apply": ([_vp, _i64, _i64, _vp, _i64, _vp, _vp], ct.c_int),
+    "pycbf_demod_apply": ([_vp, _i64, _i64, _vp, _i64, _vp, _vp, _sz, _vp], ct.c_int),
+    "pycbf_demod_workspace_bytes": ([_i64, _i64, _i64], _sz),
     "pycbf_spline_coefs": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_y2": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_workspace_bytes": ([_i64, _i32], _sz),

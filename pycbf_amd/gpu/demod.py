@@ -99,8 +99,10 @@ def fft_tukey_demod(rfdata, kmin, kmax, ks, alpha=0.25, axis=-1, shiftit=True):
         lead = tuple(x.shape[:-1])
         rows = int(np.prod(lead)) if lead else 1
         out = torch.empty(lead + (ncol,), dtype=torch.float32, device=dev)
-        rc = E.lib.pycbf_demod_apply(E.ptr(x), rows, n, E.ptr(op), ncol, E.ptr(out),
-                                     E.stream_of(dev))
+        need = int(E.lib.pycbf_demod_workspace_bytes(rows, n, ncol))
+        ws = torch.empty(max(need // 4, 1), dtype=torch.float32, device=dev) if need else None
+        rc = E.lib.pycbf_demod_apply(E.ptr(x), rows, n, E.ptr(op), ncol, E.ptr(out), E.ptr(ws),
+                                     need, E.stream_of(dev))
         E.check(rc, "pycbf_demod_apply")
         iq = torch.view_as_complex(out.view(lead + (ncol // 2, 2)))
         iq = torch.movedim(iq, -1, axis)

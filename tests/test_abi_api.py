@@ -107,11 +107,19 @@ def test_host_entry_points_report_argument_errors():
                                     None, None, None, None) != 0
     assert b"NULL" in E.lib.pycbf_last_error()
     # so does the demodulation product
-    assert E.lib.pycbf_demod_apply(None, 4, 16, None, 8, None, None) != 0
+    assert E.lib.pycbf_demod_apply(None, 4, 16, None, 8, None, None, 0, None) != 0
     assert b"NULL" in E.lib.pycbf_last_error()
-    assert E.lib.pycbf_demod_apply(None, 4, 0, None, 8, None, None) != 0
+    assert E.lib.pycbf_demod_apply(None, 4, 0, None, 8, None, None, 0, None) != 0
     assert b"n >= 1" in E.lib.pycbf_last_error()
-    assert E.lib.pycbf_demod_apply(None, 0, 16, None, 8, None, None) == 0
+    assert E.lib.pycbf_demod_apply(None, 0, 16, None, 8, None, None, 0, None) == 0
+    # one image (512 lines x 4096 samples): a split product needs a workspace
+    need = E.lib.pycbf_demod_workspace_bytes(512, 4096, 718)
+    assert need >= 2 * 512 * 718 * 4 and need % (512 * 718 * 4) == 0
+    assert E.lib.pycbf_demod_workspace_bytes(32768, 2048, 730) == 0  # tiles fill the GPU
+    buf = np.zeros(4, np.float32)
+    assert E.lib.pycbf_demod_apply(E.ptr(buf), 512, 4096, E.ptr(buf), 718, E.ptr(buf), None, 0,
+                                   None) != 0
+    assert b"workspace" in E.lib.pycbf_last_error()
 
 
 def test_adjoint_entry_point_validates_before_device_work():

@@ -15,7 +15,10 @@ U32 = 2.0 ** -24
 
 
 @pytest.mark.parametrize("rows,k,ncol", [(1, 1, 1), (63, 17, 130), (65, 100, 3), (130, 15, 64),
-                                         (200, 513, 66), (128, 128, 128), (257, 33, 129)])
+                                         (200, 513, 66), (128, 128, 128), (257, 33, 129),
+                                         # split over k (few lines, long lines): partial
+                                         # products and an ordered reduction
+                                         (64, 4096, 130), (257, 1000, 129), (5, 777, 31)])
 def test_product_within_fp32_bound(rows, k, ncol):
     """iq = rf @ opT^T from the three-way BF16 split: six exact partial
     products per term (6k summands in float32) plus the dropped cross terms
@@ -31,8 +34,10 @@ def test_product_within_fp32_bound(rows, k, ncol):
     dev = torch.device("cuda", 0)
     ad, bd = torch.as_tensor(a, device=dev), torch.as_tensor(bt, device=dev)
     out = torch.full((rows, ncol), float("nan"), dtype=torch.float32, device=dev)
-    rc = E.lib.pycbf_demod_apply(E.ptr(ad), rows, k, E.ptr(bd), ncol, E.ptr(out),
-                                 E.stream_of(dev))
+    need = int(E.lib.pycbf_demod_workspace_bytes(rows, k, ncol))
+    ws = torch.full((max(need // 4, 1),), float("nan"), dtype=torch.float32, device=dev)
+    rc = E.lib.pycbf_demod_apply(E.ptr(ad), rows, k, E.ptr(bd), ncol, E.ptr(out), E.ptr(ws),
+                                 need, E.stream_of(dev))
     E.check(rc, "pycbf_demod_apply")
     got = out.cpu().numpy().astype(np.float64)
     a64, b64 = a.astype(np.float64), bt.astype(np.float64)
