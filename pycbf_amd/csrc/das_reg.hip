@@ -26,7 +26,15 @@ namespace pycbf {
 
 namespace {
 
-constexpr int REG_TB = 8;  // transmits whose delays / weights a lane holds
+// transmits whose delays / weights a lane holds: a block of them is summed in
+// registers before the output is touched.  Per-receive images read and write
+// their output once per block, so they take 16 (75 x 128 x 256 x 2048: 4.36
+// vs 4.96 ms at 8); the summed image adds its blocks in a register and keeps
+// 8 (3.88 vs 3.98 ms at 16; 25 costs occupancy for both)
+template <int SUM>
+struct RegTB {
+    static constexpr int value = SUM == PYCBF_SUM_TX_ONLY ? 16 : 8;
+};
 
 // natural-cubic value of the table row `row` at x (0 outside the trace)
 __device__ __forceinline__ float cubic_row(const float *__restrict__ row, float x, const DasArgs &A)
@@ -57,6 +65,7 @@ __global__ __launch_bounds__(256) void k_fill(float *__restrict__ p, int64_t n, 
 template <int GEO, int SUM>
 __global__ __launch_bounds__(256) void k_das_reg(DasArgs A, int64_t ntiles, int rchunk)
 {
+    constexpr int REG_TB = RegTB<SUM>::value;
     const int lane = threadIdx.x & 63;
     const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t tile = item % ntiles;
