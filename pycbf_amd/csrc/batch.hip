@@ -10,12 +10,12 @@
 // pulls work items (request, TILE x TILE pixel tile, chunk of RC receive
 // elements) from a queue ordered by cost (largest first) through one atomic
 // counter, and exits when the queue is empty.  Within an item a lane owns a
-// pixel; it holds the delays / apodisations of 8 transmits in registers while
-// it sweeps the receive chunk (receive geometry evaluated in registers) and
-// gathers the cubic table.  DAS requests add their partial sums
-// into the zeroed image with float atomics; SLSC requests write per-receive
-// images (every (receive element, pixel) is owned by one item), which the
-// per-request SLSC launches then reduce.  Each item stamps its request's
+// pixel; it holds the delays / apodisations of 8 transmits (16 for SLSC
+// requests) in registers while it sweeps the receive chunk (receive geometry
+// evaluated in registers) and gathers the cubic table.  DAS requests add
+// their partial sums into the zeroed image with float atomics; SLSC requests
+// write per-receive images (every (receive element, pixel) is owned by one
+// item), which the per-request SLSC launches then reduce.  Each item stamps its request's
 // completion time (wall clock) for per-request latencies.  Every launch of a
 // run is a kernel (no memset), so a run can be captured in a graph.
 #include <math.h>
