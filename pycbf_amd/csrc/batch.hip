@@ -15,9 +15,10 @@
 // evaluated in registers) and gathers the cubic table.  DAS requests add
 // their partial sums into the zeroed image with float atomics; SLSC requests
 // write per-receive images (every (receive element, pixel) is owned by one
-// item), which the per-request SLSC launches then reduce.  Each item stamps its request's
-// completion time (wall clock) for per-request latencies.  Every launch of a
-// run is a kernel (no memset), so a run can be captured in a graph.
+// item), which the per-request SLSC launches then reduce.  Each item stamps
+// its request's completion time (wall clock) for per-request latencies.
+// Every launch of a run is a kernel (no memset), so a run can be captured in
+// a graph.
 #include <math.h>
 
 #include <algorithm>
