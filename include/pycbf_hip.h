@@ -335,7 +335,7 @@ void pycbf_batch_destroy(pycbf_batch *batch);
 /* Device workspace bytes a batch needs (spline tables of every channel, the
  * request table, the work items of the largest configuration, counters). */
 size_t pycbf_batch_workspace_bytes(const pycbf_batch *batch);
-/* Picks pixel tile (8, 16 or 32 square) and receive chunk (16, 32 or 64) and
+/* Picks pixel tile (8, 16 or 32 square) and receive chunk (16, 32, 64 or 128) and
  * uploads the request table and the cost-ordered work items; synchronises. */
 int pycbf_batch_configure(pycbf_batch *batch, int tile, int rchunk, void *workspace,
                           size_t workspace_bytes, void *stream);
@@ -344,9 +344,10 @@ int pycbf_batch_configure(pycbf_batch *batch, int tile, int rchunk, void *worksp
  * SLSC of the SLSC requests. */
 int pycbf_batch_run(pycbf_batch *batch, const float *rf_all, void *workspace,
                     size_t workspace_bytes, void *stream);
-/* Times the persistent gather for every (tile, rchunk) pair (reps runs each,
- * after one full run that builds the tables), configures the fastest and
- * returns it; times_ms (9 entries, tile-major) may be NULL.  Synchronises. */
+/* Times the persistent gather for every tile (8, 16, 32) and receive chunk
+ * (32, 64, 128) (reps runs each, after one full run that builds the tables),
+ * configures the fastest and returns it; times_ms (9 entries, tile-major) may
+ * be NULL.  Synchronises. */
 int pycbf_batch_autotune(pycbf_batch *batch, const float *rf_all, int reps, void *workspace,
                          size_t workspace_bytes, void *stream, int *best_tile, int *best_rchunk,
                          float *times_ms);

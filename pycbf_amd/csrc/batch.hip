@@ -330,7 +330,11 @@ static int launch_gather(const BatchPlan &P, const BatchPtrs &b, hipStream_t s)
     case 1664: launch_gather_t<16, 64>(P, b, s); break;
     case 3216: launch_gather_t<32, 16>(P, b, s); break;
     case 3232: launch_gather_t<32, 32>(P, b, s); break;
-    default: launch_gather_t<32, 64>(P, b, s); break;
+    case 3264: launch_gather_t<32, 64>(P, b, s); break;
+    case 928: launch_gather_t<8, 128>(P, b, s); break;     // 8 * 100 + 128
+    case 1728: launch_gather_t<16, 128>(P, b, s); break;   // 16 * 100 + 128
+    case 3328: launch_gather_t<32, 128>(P, b, s); break;   // 32 * 100 + 128
+    default: return fail(PYCBF_EINVAL, "unknown gather configuration");
     }
     return check_launch("batch gather");
 }
@@ -415,7 +419,7 @@ int pycbf_batch_create(const pycbf_batch_req *reqs, int nreq, pycbf_batch **out)
     P.nch = chans;
     P.npix_das = das_pix;
     for (int t : {8, 16, 32})
-        for (int c : {16, 32, 64})
+        for (int c : {16, 32, 64, 128})
             P.max_items = (int)std::max<int64_t>(P.max_items, items_for(P.reqs, t, c, nullptr));
     *out = new pycbf_batch{P};
     return 0;
@@ -432,8 +436,9 @@ int pycbf_batch_configure(pycbf_batch *b, int tile, int rchunk, void *workspace,
                           size_t workspace_bytes, void *stream)
 {
     if (!b) return fail(PYCBF_EINVAL, "NULL batch");
-    if ((tile != 8 && tile != 16 && tile != 32) || (rchunk != 16 && rchunk != 32 && rchunk != 64))
-        return fail(PYCBF_EINVAL, "tile must be 8, 16 or 32 and rchunk 16, 32 or 64");
+    if ((tile != 8 && tile != 16 && tile != 32) ||
+        (rchunk != 16 && rchunk != 32 && rchunk != 64 && rchunk != 128))
+        return fail(PYCBF_EINVAL, "tile must be 8, 16 or 32 and rchunk 16, 32, 64 or 128");
     if (!workspace || workspace_bytes < batch_layout(b->plan).total)
         return fail(PYCBF_EINVAL, "workspace too small");
     b->plan.tile = tile;
@@ -457,7 +462,10 @@ int pycbf_batch_autotune(pycbf_batch *b, const float *rf_all, int reps, void *wo
 {
     if (!b || !rf_all || reps < 1) return fail(PYCBF_EINVAL, "bad autotune arguments");
     hipStream_t s = (hipStream_t)stream;
-    const int tiles[3] = {8, 16, 32}, chunks[3] = {16, 32, 64};
+    // receive chunks of 16 were the slowest at every tile on the 64-request
+    // batch of 128..512-square grids and 64..256 receive elements (20.9 /
+    // 18.4 / 17.8 ms for 16 / 32 / 64 at tile 8); they stay configurable
+    const int tiles[3] = {8, 16, 32}, chunks[3] = {32, 64, 128};
     float best = 1e30f;
     int bt = 16, bc = 32;
     hipEvent_t e0, e1;

@@ -176,7 +176,7 @@ class RaggedBatch:
         self.tile, self.rchunk = bt.value, bc.value
         self._graph = None
         grid = {(t, c): times[i * 3 + j] for i, t in enumerate((8, 16, 32))
-                for j, c in enumerate((16, 32, 64))}
+                for j, c in enumerate((32, 64, 128))}
         return self.tile, self.rchunk, grid
 
     def _launch(self):

@@ -16,7 +16,9 @@ def _requests(seed=0, nt=768):
     from pycbf_amd.gpu.batch import BatchRequest
     rng = np.random.default_rng(seed)
     shapes = [(16, 24, 1, 8, "das"), (40, 33, 3, 16, "das"), (24, 24, 5, 24, "slsc"),
-              (9, 70, 2, 12, "das"), (32, 16, 4, 40, "slsc"), (48, 40, 7, 20, "das")]
+              (9, 70, 2, 12, "das"), (32, 16, 4, 40, "slsc"), (48, 40, 7, 20, "das"),
+              # 136 receive elements: a ragged second chunk at every receive chunk
+              (12, 10, 2, 136, "das")]
     reqs = []
     for k, (nx, nz, ntx, nrx, kind) in enumerate(shapes):
         xele = sim.linear_array(nrx)
@@ -81,7 +83,7 @@ def setup():
 def test_batch_matches_single_requests(setup):
     import torch
     batch, reqs, refs = setup
-    print("ragged batch of 6 requests, default configuration", flush=True)
+    print("ragged batch of 7 requests, default configuration", flush=True)
     batch.run()
     torch.cuda.synchronize()
     _check(batch, reqs, refs)
@@ -90,7 +92,7 @@ def test_batch_matches_single_requests(setup):
 
 
 @pytest.mark.parametrize("tile", [8, 16, 32])
-@pytest.mark.parametrize("rchunk", [16, 32, 64])
+@pytest.mark.parametrize("rchunk", [16, 32, 64, 128])
 def test_batch_every_configuration(setup, tile, rchunk):
     import torch
     batch, reqs, refs = setup
