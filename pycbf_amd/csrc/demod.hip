@@ -22,7 +22,9 @@
 // a 64 x 64 quarter as 2 x 2 blocks of 32 x 32.  k advances in slices of 32:
 // the workgroup loads the next slice's rf and opT tiles into registers while
 // the current one (three BF16 planes each, in LDS) feeds the MFMAs; edges are
-// zero-filled.
+// zero-filled.  48 KB of LDS and 168 registers per thread give three
+// workgroups per CU (three waves per SIMD), so one wave's splitting overlaps
+// another's MFMAs.
 #include <algorithm>
 
 #include "common.h"
@@ -37,7 +39,17 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 constexpr int DM = 128;     // workgroup tile rows (lines) and columns
 constexpr int DK = 32;      // k slice
-constexpr int DP = DK + 8;  // LDS row pitch in BF16 (80 bytes: rows stay 16-byte aligned)
+constexpr int DP = DK;      // LDS row pitch in BF16 (64 bytes, four 16-byte chunks)
+
+// 16-byte chunk c of LDS row r sits at chunk c ^ ((r >> 1) & 3): the eight
+// rows one ds_read_b128 phase reads (8 lanes, rows j .. j+7, one chunk) then
+// cover all 32 banks once, where an unswizzled 64-byte pitch puts them 4-way
+// on the same banks.  The unpadded pitch keeps a workgroup's planes at 48 KB
+// (three workgroups per CU).
+__device__ __forceinline__ int lds_off(int row, int k)
+{
+    return row * DP + ((((k >> 3) ^ (row >> 1)) & 3) << 3) + (k & 7);
+}
 
 // rows [row0, row0 + 128) x k [k0, k0 + 32) of a (nrow x ld) float32 matrix,
 // zero from column kend on: thread t holds rows t/8 + 32q (q < 4), k 4(t%8) .. +3
@@ -82,7 +94,7 @@ __device__ __forceinline__ void store_split(__bf16 *__restrict__ s, const F4u (&
             p1[e] = h1;
             p2[e] = (__bf16)r2;
         }
-        const int o = ((t >> 3) + 32 * q) * DP + kc;
+        const int o = lds_off((t >> 3) + 32 * q, kc);
         *reinterpret_cast<bf16x4_t *>(s + o) = p0;
         *reinterpret_cast<bf16x4_t *>(s + DM * DP + o) = p1;
         *reinterpret_cast<bf16x4_t *>(s + 2 * DM * DP + o) = p2;
@@ -92,7 +104,7 @@ __device__ __forceinline__ void store_split(__bf16 *__restrict__ s, const F4u (&
 
 // blockIdx.z: the slice [z kc, (z+1) kc) of k; with several slices C is the
 // (nsplit, R, N) array of partial products
-__global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A, int64_t R, int K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_demod_gemm(const float *__restrict__ A, int64_t R, int K,
                                                     const float *__restrict__ Bt, int N, int kc,
                                                     float *__restrict__ C)
 {
@@ -131,8 +143,8 @@ __global__ __launch_bounds__(256) void k_demod_gemm(const float *__restrict__ A,
             for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
                 for (int p = 0; p < 3; ++p) {
-                    const int oa = p * DM * DP + (wr + 32 * bi + j) * DP + kk + 8 * h;
-                    const int ob = p * DM * DP + (wc + 32 * bi + j) * DP + kk + 8 * h;
+                    const int oa = p * DM * DP + lds_off(wr + 32 * bi + j, kk + 8 * h);
+                    const int ob = p * DM * DP + lds_off(wc + 32 * bi + j, kk + 8 * h);
                     a[bi][p] = *reinterpret_cast<const bf16x8_t *>(As + oa);
                     b[bi][p] = *reinterpret_cast<const bf16x8_t *>(Bs + ob);
                 }
