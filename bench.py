@@ -331,9 +331,11 @@ def sos_leg(dev, steps):
         loss = prob.loss(cmap(), e0)
         loss.backward()
         opt.step()
-        losses.append(float(loss.detach()))
+        # kept on the device: reading it here would stall the host every step
+        losses.append(loss.detach())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    losses = [float(v) for v in torch.stack(losses).cpu()]
     E.lib.pycbf_profile_enable(0)
     kern = {}
     for k in ("ray_delays", "ray_delays_bwd", "das_gather", "das_bwd_tau", "spline_coefs"):
