@@ -80,12 +80,31 @@ def measured_traffic(frames):
     return vals["traffic_bytes_per_launch"]
 
 
+def cpu_info():
+    """Host CPU of this run: model name, logical CPUs, CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "cpu_count": os.cpu_count(), "usable": usable}
+
+
 def cpu_baseline(frames, params, stride, threads):
     """Reference C cubic DAS (oracle/_ref, pycbf/cpu/pycbf.c) -- or the oracle
     restatement when the reference build is absent -- on the given host frames
     over a strided pixel subset, channels split over `threads` host threads
-    with private accumulators.  Returns (Gsamples/s, threads, kind, sample
-    description)."""
+    with private accumulators (the reference class's worker pool,
+    pycbf/cpu/__core_bf_classes__.py:99-114).  Returns (Gsamples/s, threads,
+    kind, sample description, per-frame float64 images)."""
     import ctypes as ct
 
     import oracle
@@ -99,32 +118,79 @@ def cpu_baseline(frames, params, stride, threads):
     kind = "reference" if R is not None else "port"
     fn = R.beamform_cubic if R is not None else oracle.lib().orc_bf_cubic
     pf = ct.POINTER(ct.c_float)
-    outs = [np.zeros(npx, np.float32) for _ in range(threads)]
     dt = float(np.float32(1.0 / 40e6))
     chans = [(t, r) for t in range(NTX) for r in range(NRX)]
 
-    def work(k, rf0):
-        o = outs[k].ctypes.data_as(pf)
+    def work(o, k, rf0):
+        op = o.ctypes.data_as(pf)
         for idx in range(k, len(chans), threads):
             t, r = chans[idx]
             fn(0.0, dt, NT, rf0[t, r].ctypes.data_as(pf), npx, 1e-2,
                tautx[t].ctypes.data_as(pf), apodtx[t].ctypes.data_as(pf),
-               taurx[r].ctypes.data_as(pf), apodrx[r].ctypes.data_as(pf), o)
+               taurx[r].ctypes.data_as(pf), apodrx[r].ctypes.data_as(pf), op)
 
     wall = 0.0
+    images = []
     for rf0 in frames:
         rf0 = np.ascontiguousarray(rf0, np.float32)
-        ths = [threading.Thread(target=work, args=(k, rf0)) for k in range(threads)]
+        outs = [np.zeros(npx, np.float32) for _ in range(threads)]
+        ths = [threading.Thread(target=work, args=(outs[k], k, rf0)) for k in range(threads)]
         t0 = time.perf_counter()
         for th in ths:
             th.start()
         for th in ths:
             th.join()
         wall += time.perf_counter() - t0
+        images.append(np.sum(np.stack(outs).astype(np.float64), axis=0))
     samples = NTX * NRX * npx * len(frames)
     desc = (f"{len(frames)} frames, {NTX}x{NRX} channels, every {stride}th of {NX * NZ} pixels "
-            f"({npx} px, {samples / 1e9:.1f} G samples), cubic, {wall:.2f} s wall")
-    return samples / wall / 1e9, threads, kind, desc
+            f"({npx} px, {samples / 1e9:.2f} G samples), cubic, {wall:.2f} s wall")
+    return samples / wall / 1e9, threads, kind, desc, images
+
+
+def count_contributing(bf, thresh):
+    """(tx, rx, px) samples of one frame that contribute: weight passes the
+    threshold and the delay lies on the trace (the others cost the staged
+    gather no table reads)."""
+    import torch
+    ttx, atx, trx, arx = bf.tables()
+    tmax = float(np.float32(0.0) + np.float32(1.0 / 40e6) * np.float32(NT - 1))
+    n = 0
+    for t in range(ttx.shape[0]):
+        w = (atx[t][None, :] * arx).abs() >= thresh
+        x = ttx[t][None, :] + trx
+        n += int((w & (x >= 0.0) & (x <= tmax)).sum())
+    del ttx, atx, trx, arx
+    torch.cuda.empty_cache()
+    return n
+
+
+def time_variant(env, value, params, rf, ref_out, steps):
+    """Times the same 16-frame step with another gather selected by an
+    environment switch of the library; returns (ms per step, max error
+    relative to the headline images' peak)."""
+    import torch
+
+    from pycbf_amd.gpu import SyntheticBeamformer
+    os.environ[env] = value
+    try:
+        bfv = SyntheticBeamformer(**params, t0=0.0, dt=1.0 / 40e6, nt=NT,
+                                  interp={"kind": "cubic"}, sumtype="tx_and_rx",
+                                  device=rf.device)
+        outv = torch.empty_like(ref_out)
+        bfv(rf, out_as_numpy=False, buffer=outv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bfv(rf, out_as_numpy=False, buffer=outv)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    finally:
+        del os.environ[env]
+    err = float((outv - ref_out).abs().max() / ref_out.abs().max())
+    del bfv, outv
+    torch.cuda.empty_cache()
+    return ms, err
 
 
 SLSC_NXO, SLSC_NZO, SLSC_SUB, SLSC_LAGS, SLSC_KWIN = 256, 256, 8, list(range(1, 26)), 8
@@ -517,7 +583,7 @@ def main():
     ap.add_argument("--sos-steps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch leg")
     ap.add_argument("--batch-runs", type=int, default=5)
-    ap.add_argument("--batch-speckle", type=int, default=2000)
+    ap.add_argument("--batch-speckle", type=int, default=10_000)
     ap.add_argument("--slsc-reps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the channel-data adjoint leg")
     ap.add_argument("--adjoint-reps", type=int, default=10)
@@ -586,21 +652,54 @@ def main():
     t_ms, t_n = E.profile_read("synthetic_tables")
     gather_ms = g_ms / max(g_n, 1)
     npx = NX * NZ
-    algo_bytes = (16.0 * samples + 4.0 * NTX * NRX * NT * args.frames + 4.0 * 2 * npx
-                  + 4.0 * npx * args.frames)
-    achieved = algo_bytes / (gather_ms * 1e-3) / 1e9
     log(f"gather {gather_ms:.3f} ms/launch, spline {s_ms / max(s_n, 1):.3f} ms, "
         f"tables {t_ms / max(t_n, 1):.3f} ms")
+    # HBM roofline of the gather on its compulsory bytes: the (y, h*y2) spline
+    # table read once (8 B per entry, nt + 1 entries per channel), the pixel
+    # positions and processing order, the images written
+    table_bytes = 8.0 * args.frames * NTX * NRX * (NT + 1)
+    compulsory = table_bytes + 12.0 * npx + 4.0 * npx * args.frames
+    hbm_achieved = compulsory / (gather_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.frames)
+    # on-chip: the stencil reads (16 B per contributing sample and frame) the
+    # gather serves from LDS, against the measured LDS read bandwidth
+    contrib = count_contributing(bf, 1e-2)
+    peaks = {k: round(E.measure_bandwidth(k, dev), 1) for k in ("hbm", "l2", "lds")}
+    log(f"measured read bandwidth GB/s: {peaks}; contributing samples per frame {contrib} "
+        f"({contrib / (NTX * NRX * npx):.3f} of all)")
+    lds_bytes = 16.0 * contrib * args.frames
+    lds_achieved = lds_bytes / (gather_ms * 1e-3) / 1e9
 
-    cpu = None
+    # plain-GPU comparators on the same workload and data
+    plain_ms, plain_err = time_variant("PYCBF_DAS_PLAIN", "1", params, rf, out, 2)
+    direct_ms, direct_err = time_variant("PYCBF_DAS_LDS", "0", params, rf, out, 2)
+    log(f"thread-per-sample comparator {plain_ms:.2f} ms/step (err {plain_err:.2e}); direct "
+        f"gather {direct_ms:.2f} ms/step (err {direct_err:.2e})")
+
+    cpu = parity = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        frames = (rf[f].cpu().numpy() for f in range(min(args.cpu_frames, args.frames)))
-        rate, used, kind, desc = cpu_baseline(list(frames), params, args.cpu_stride, threads)
+        info = cpu_info()
+        threads = min(16, info["usable"])
+        nf = min(args.cpu_frames, args.frames)
+        frames = [rf[f].cpu().numpy() for f in range(nf)]
+        rate, used, kind, desc, images = cpu_baseline(frames, params, args.cpu_stride, threads)
+        rate1, _, _, desc1, _ = cpu_baseline(frames[:1], params, 16, 1)
         cpu = {"value": round(rate, 4), "unit": "Gsamples/s", "cores": used, "kind": kind,
-               "sample": desc}
-        log(f"cpu baseline: {rate:.4f} Gsamples/s on {used} threads ({kind})")
+               "sample": desc, "cpu_model": info["model"], "cpu_count": info["cpu_count"],
+               "cpus_usable": info["usable"],
+               "one_core": {"value": round(rate1, 4), "unit": "Gsamples/s", "sample": desc1}}
+        log(f"cpu baseline: {rate:.4f} Gsamples/s on {used} threads ({kind}); one core "
+            f"{rate1:.4f}")
+        # parity of the timed GPU frames with the CPU images of the same frames
+        sub = torch.arange(0, npx, args.cpu_stride, device=dev)
+        gpu = out[:nf, sub].double().cpu().numpy()
+        errs = [float(np.abs(gpu[f] - images[f]).max() / np.abs(images[f]).max())
+                for f in range(nf)]
+        parity = {"frames": nf, "pixels_per_frame": int(sub.numel()),
+                  "max_err_rel_peak": float(f"{max(errs):.3g}"), "bar": 1e-5,
+                  "against": f"cpu_baseline images ({kind} C)"}
+        log(f"parity: max error {max(errs):.3g} of the peak over {nf} frames")
+        del frames
 
     slsc_res = demod_res = None
     if not args.no_slsc:
@@ -628,11 +727,31 @@ def main():
                                f"nt={NT}, {args.frames} frames, sumtype tx_and_rx",
                    "ntx": NTX, "nrx": NRX, "nt": NT, "grid": [NX, NZ], "frames": args.frames,
                    "interp": "cubic"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "roofline": {"bound": "hbm", "achieved": round(hbm_achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "kernel": "das_gather",
-                     "kernel_ms": round(gather_ms, 4)},
+                     "kernel_ms": round(gather_ms, 4), "compulsory_bytes": compulsory,
+                     "bytes": "spline table read once + pixel positions and order + images"},
+        "roofline_onchip": {"bound": "lds", "achieved": round(lds_achieved, 1),
+                            "peak": peaks["lds"], "unit": "GB/s",
+                            "frac": round(lds_achieved / peaks["lds"], 4),
+                            "bytes": "16 B per contributing sample and frame",
+                            "contributing_samples_per_frame": contrib,
+                            "all_samples_per_frame": NTX * NRX * npx},
+        "peaks_measured_gbps": peaks,
+        "plain_gpu": {"value": round(samples / (plain_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
+                      "ms_per_step": round(plain_ms, 3),
+                      "max_err_rel_peak_vs_headline": float(f"{plain_err:.3g}"),
+                      "kernel": "thread per (frame, tx, rx, px) sample with one float atomic "
+                                "each: pycbf's GPU launch scheme (PYCBF_DAS_PLAIN=1)"},
+        "direct_gpu": {"value": round(samples / (direct_ms * 1e-3) / 1e9, 2),
+                       "unit": "Gsamples/s", "ms_per_step": round(direct_ms, 3),
+                       "max_err_rel_peak_vs_headline": float(f"{direct_err:.3g}"),
+                       "kernel": "lane per pixel over every channel, delay tables, direct "
+                                 "global gathers (PYCBF_DAS_LDS=0)"},
+        "speedup_vs_plain_gpu": round(plain_ms / ms_step, 2),
         "cpu_baseline": cpu,
+        "parity": parity,
         "phases_ms": {"das_gather": round(gather_ms, 4),
                       "spline_coefs": round(s_ms / max(s_n, 1), 4),
                       "synthetic_tables": round(t_ms / max(t_n, 1), 4)},

@@ -61,6 +61,13 @@ extern "C" {
 #define PYCBF_SLSC_MFMA_FP16 1
 #define PYCBF_SLSC_MFMA_BF16 2
 
+/* The `thresh` that reproduces pycbf's GPU weight rule |apodtx*apodrx| > 0.01
+ * (pycbf/gpu/__engines__.cu:8,564,678,762,851: a float compared with the
+ * double 0.01) under this library's rule |w| >= thresh: the smallest float
+ * above 0.01.  pycbf's CPU kernels use |w| >= 1e-2f (pycbf/cpu/pycbf.c:201,267),
+ * the default of the Python classes. */
+#define PYCBF_GPU_REFERENCE_THRESH 0.010000000707805157f
+
 /* floats per transmit / receive element in the synthetic geometry packs */
 #define PYCBF_TXPACK_FLOATS 16
 #define PYCBF_RXPACK_FLOATS 8
@@ -355,6 +362,16 @@ int pycbf_batch_autotune(pycbf_batch *batch, const float *rf_all, int reps, void
  * (device wall clock); synchronous readback. */
 int pycbf_batch_latencies(const pycbf_batch *batch, const void *workspace,
                           size_t workspace_bytes, double *latency_ms);
+
+/* Measured read bandwidth of the device in GB/s (no reference counterpart:
+ * the roofline denominators of the benchmark).  kind 0: one streaming read of
+ * buf (bytes, larger than the caches) per rep; kind 1: 8192 workgroups each
+ * read the region buf (bytes, small enough for one XCD's L2) 8 times; kind 2:
+ * LDS reads of 16 bytes per lane, conflict-free (buf unused).  sink: device
+ * scratch of at least 8192 * 256 * 4 bytes.  One untimed warm-up launch, then
+ * reps timed launches with HIP events on stream; synchronises. */
+int pycbf_measure_bandwidth(int kind, const void *buf, size_t bytes, int reps, void *sink,
+                            size_t sink_bytes, double *gbps, void *stream);
 
 /* Per-kernel timing: when enabled, instrumented launches record HIP events on
  * their stream; pycbf_profile_read synchronises them and returns the summed

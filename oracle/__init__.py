@@ -228,16 +228,21 @@ def das_tabbed(rf, t0, dt, tautx, apodtx, taurx, apodrx, kind="cubic", usf=1, th
 
 
 def das_tabbed_mt(rf, t0, dt, tautx, apodtx, taurx, apodrx, kind="cubic", usf=1, thresh=1e-2,
-                  nthreads=0):
-    """Multi-threaded tx_and_rx DAS (CPU baseline); returns (image, threads_used)."""
+                  nthreads=0, S=None):
+    """Multi-threaded tx_and_rx DAS (channels split over OpenMP threads with
+    private accumulators, summed in double); returns (image, threads_used)."""
     rf = _f32(rf)
     ntx, nrx, nt = rf.shape
     tautx, apodtx, taurx, apodrx = (_f32(a) for a in (tautx, apodtx, taurx, apodrx))
     np_ = tautx.shape[1]
     out = np.empty(np_, np.float32)
+    k, Sp = 0, None
+    if kind == "korder_cubic":
+        S = _f32(S)
+        k, Sp = S.shape[0], _ptr(S)
     used = lib().orc_das_tabbed_mt(_ptr(rf), ntx, nrx, nt, t0, dt, _ptr(tautx), _ptr(apodtx),
-                                   _ptr(taurx), _ptr(apodrx), np_, KINDS[kind], int(usf), None,
-                                   0, thresh, _ptr(out), int(nthreads))
+                                   _ptr(taurx), _ptr(apodrx), np_, KINDS[kind], int(usf), Sp,
+                                   k, thresh, _ptr(out), int(nthreads))
     return out, used
 
 

@@ -124,13 +124,14 @@ def upsample(rf: torch.Tensor, t0: float, dt: float, usf: int) -> torch.Tensor:
 
 
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="tx_and_rx",
-               thresh=0.0, x32=True, S=None, usf=1):
+               thresh=0.0, x32=True, S=None, usf=1, y2=None):
     """rf (nfr, ntx, nrx, n); tables (ntx, np) / (nrx, np); returns float64
     with the sumtype layout of the GPU beamformers (frames first).  x32 sums
     the delays in float32 as the kernels do (same sample indices); x32=False
     keeps the whole function in float64 (finite-difference checks).  S: the
     (k, k) derivative operator of korder_cubic; usf: the upsampling factor
-    of nearest."""
+    of nearest; y2: natural_y2(rf, dt) when the caller already has it (cubic,
+    repeated evaluations on the same channel data)."""
     rf = rf.to(torch.float64)
     if kind == "nearest" and usf > 1:
         # nearest on the upsampled trace at dt / usf (float32, as the kernels)
@@ -190,7 +191,8 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, kind="cubic", sumtype="
         b = (x - (t0 + dt * jf)) / dt
         v = a * take(y, j) + b * take(y, j + 1)
         if kind == "cubic":
-            y2 = torch.cat([natural_y2(rf, dt), zpad], dim=-1)
+            y2 = torch.cat([natural_y2(rf, dt) if y2 is None else y2.to(rf.dtype), zpad],
+                           dim=-1)
             v = v + (dt * dt / 6.0) * ((a ** 3 - a) * take(y2, j) + (b ** 3 - b) * take(y2, j + 1))
     s = v * (w * keep * inside)[None]                        # (nfr, ntx, nrx, np)
     if sumtype == "none":

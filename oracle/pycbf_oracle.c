@@ -475,6 +475,7 @@ void orc_synthetic_tables(int ndim, int ntx, const float *ovectx, const float *n
 void orc_indices(const float *tautx, const float *taurx, int ntx, int nrx, int64_t np, float t0,
                  float dt, int n, int nearest, int32_t *idx)
 {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int t = 0; t < ntx; ++t)
         for (int r = 0; r < nrx; ++r)
             for (int64_t i = 0; i < np; ++i) {

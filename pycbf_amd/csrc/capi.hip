@@ -254,6 +254,7 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
         break;
     }
     }
+    if (das_use_plain()) return launch_das_plain(plan.gk, a, s);
     if (das_reg_applies(plan.gk, nfr, sumflag)) return launch_das_reg(a, s);
     float *part = nullptr;
     if (plan.part) {
@@ -281,6 +282,68 @@ static float acos_threshold(float ala)
     float T = (float)c;
     if ((double)T < c) T = nextafterf(T, INFINITY);
     return T;
+}
+
+// A plan whose partial images do not fit `cap` bytes takes fewer, larger
+// outer groups (down to one, no partials); the images are the same sums in
+// another order.
+static void cap_partials(DasPlan &p, int64_t nfr, int64_t np, size_t cap)
+{
+    if (p.total <= cap || p.part == 0) return;
+    const size_t rest = p.total - p.part;
+    const size_t per = (size_t)nfr * (size_t)np * sizeof(float);
+    int64_t s = (cap > rest + 256) ? (int64_t)((cap - rest - 256) / per) : 0;
+    if (s >= p.nsplit) return;
+    if (s <= 1) {
+        p.nsplit = 1;
+        p.part = 0;
+    } else {
+        p.nsplit = (int)s;
+        p.part = align_up((size_t)s * per, 256);
+    }
+    p.total = rest + p.part;
+}
+
+int das_synthetic_in(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                     const float *txpack, const float *rxpack, float c0, const float *pvec,
+                     const int32_t *perm, int64_t np, int interp_kind, int interp_param,
+                     const float *S, float thresh, int sumflag, float *out, void *buf,
+                     size_t cap, bool shrink_split, hipStream_t s)
+{
+    int rc = validate_common(info, rf, nfr, np, interp_kind, interp_param, S, sumflag, out);
+    if (rc) return rc;
+    if (info->ndim != 2 && info->ndim != 3)
+        return fail(PYCBF_EINVAL, "synthetic geometry needs ndim 2 or 3");
+    if (!txpack || !rxpack || !pvec) return fail(PYCBF_EINVAL, "NULL geometry pointer");
+    if (!(c0 > 0.0f) || !isfinite(c0)) return fail(PYCBF_EINVAL, "c0 must be finite and > 0");
+    DasPlan plan = plan_das(info, nfr, np, interp_kind, interp_param, 1, sumflag);
+    if (shrink_split) cap_partials(plan, nfr, np, cap);
+    if (cap < plan.total || !buf)
+        return fail(PYCBF_EINVAL, "%s of %zu bytes needed, %zu given",
+                    shrink_split ? "scratch" : "workspace", plan.total, cap);
+    Carver cv(buf, cap);
+    if (plan.bounds) {
+        // staged gather: delays and weights are evaluated in registers
+        SynGeo g;
+        g.txpack = txpack;
+        g.rxpack = rxpack;
+        g.pvec = pvec;
+        g.c0 = c0;
+        g.ndim = info->ndim;
+        return run_das(info, rf, nfr, nullptr, nullptr, nullptr, nullptr, perm, np, interp_kind,
+                       interp_param, S, thresh, sumflag, out, plan, cv, s, g);
+    }
+    const size_t tb = plan.tabs / 4;
+    float *ttx = (float *)cv.take(tb);
+    float *atx = (float *)cv.take(tb);
+    float *trx = (float *)cv.take(tb);
+    float *arx = (float *)cv.take(tb);
+    if (!ttx || !atx || !trx || !arx) return fail(PYCBF_EINVAL, "workspace too small");
+    rc = launch_synth_tables(info->ndim, (int)info->ntx, (int)info->nrx, txpack, rxpack, c0, pvec,
+                             np, ttx, atx, trx, arx, s);
+    if (rc) return rc;
+    return run_das(info, rf, nfr, ttx, atx, trx, arx, perm, np, interp_kind, interp_param, S,
+                   thresh, sumflag, out, plan, cv, s);
 }
 
 }  // namespace pycbf
@@ -418,39 +481,9 @@ int pycbf_das_synthetic(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                         const float *S, float thresh, int sumflag, float *out, void *workspace,
                         size_t workspace_bytes, void *stream)
 {
-    int rc = validate_common(info, rf, nfr, np, interp_kind, interp_param, S, sumflag, out);
-    if (rc) return rc;
-    if (info->ndim != 2 && info->ndim != 3)
-        return fail(PYCBF_EINVAL, "synthetic geometry needs ndim 2 or 3");
-    if (!txpack || !rxpack || !pvec) return fail(PYCBF_EINVAL, "NULL geometry pointer");
-    if (!(c0 > 0.0f) || !isfinite(c0)) return fail(PYCBF_EINVAL, "c0 must be finite and > 0");
-    const DasPlan plan = plan_das(info, nfr, np, interp_kind, interp_param, 1, sumflag);
-    if (workspace_bytes < plan.total || !workspace)
-        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
-                    workspace_bytes);
-    Carver cv(workspace, workspace_bytes);
-    hipStream_t s = (hipStream_t)stream;
-    if (plan.bounds) {
-        // staged gather: delays and weights are evaluated in registers
-        SynGeo g;
-        g.txpack = txpack;
-        g.rxpack = rxpack;
-        g.pvec = pvec;
-        g.c0 = c0;
-        g.ndim = info->ndim;
-        return run_das(info, rf, nfr, nullptr, nullptr, nullptr, nullptr, perm, np, interp_kind,
-                       interp_param, S, thresh, sumflag, out, plan, cv, s, g);
-    }
-    const size_t tb = plan.tabs / 4;
-    float *ttx = (float *)cv.take(tb);
-    float *atx = (float *)cv.take(tb);
-    float *trx = (float *)cv.take(tb);
-    float *arx = (float *)cv.take(tb);
-    if (!ttx || !atx || !trx || !arx) return fail(PYCBF_EINVAL, "workspace too small");
-    rc = pycbf_synthetic_tables(info, txpack, rxpack, c0, pvec, np, ttx, atx, trx, arx, stream);
-    if (rc) return rc;
-    return run_das(info, rf, nfr, ttx, atx, trx, arx, perm, np, interp_kind, interp_param, S,
-                   thresh, sumflag, out, plan, cv, s);
+    return das_synthetic_in(info, rf, nfr, txpack, rxpack, c0, pvec, perm, np, interp_kind,
+                            interp_param, S, thresh, sumflag, out, workspace, workspace_bytes,
+                            false, (hipStream_t)stream);
 }
 
 int pycbf_tile_order(const float *pvec, int64_t np, int ndim, int32_t *perm)

@@ -97,6 +97,10 @@ int launch_das_lds(int gk, const DasArgs &a, int nsplit, float *partial, hipStre
 // for one frame summed over tx and rx, or over tx only (das_reg.hip).
 bool das_reg_applies(int gk, int64_t nfr, int sumflag);
 int launch_das_reg(const DasArgs &a, hipStream_t s);
+// Thread-per-sample gather with one float atomic per contributing sample
+// (pycbf's own GPU launch scheme), the comparator selected by PYCBF_DAS_PLAIN=1.
+bool das_use_plain();
+int launch_das_plain(int gk, const DasArgs &a, hipStream_t s);
 // Delay min/max per (pixel tile of 256, element): tx elements then rx.
 int launch_tile_bounds(const DasArgs &a, float *bounds, hipStream_t s);
 inline int64_t das_tiles(int64_t np) { return (np + 255) / 256; }
@@ -108,6 +112,13 @@ bool das_use_lds();
 int das_frames_per_pass(int64_t nfr);
 // Whether transmits are the outer (one element per workgroup) axis.
 bool das_outer_is_tx(int sumflag, int64_t ntx, int64_t nrx);
+// Body of pycbf_das_synthetic on a caller buffer of `cap` bytes; with
+// shrink_split the outer split's partial images shrink until the plan fits.
+int das_synthetic_in(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                     const float *txpack, const float *rxpack, float c0, const float *pvec,
+                     const int32_t *perm, int64_t np, int interp_kind, int interp_param,
+                     const float *S, float thresh, int sumflag, float *out, void *buf,
+                     size_t cap, bool shrink_split, hipStream_t s);
 int launch_synth_tables(int ndim, int ntx, int nrx, const float *txpack, const float *rxpack,
                         float c0, const float *pvec, int64_t np, float *ttx, float *atx,
                         float *trx, float *arx, hipStream_t s);

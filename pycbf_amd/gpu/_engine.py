@@ -35,6 +35,10 @@ SUMTYPE = {"none": 0, "tx_only": 1, "rx_only": 2, "tx_and_rx": 3}
 DMASTYPE = {"ssr": 0, "power": 1, "pssr": 2}
 TABS = {"plane": 0, "divergent": 1, "focused": 2}
 SLSC_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+# thresh reproducing pycbf's GPU rule |w| > 0.01 (a float against the double
+# 0.01, pycbf/gpu/__engines__.cu:8,564) under |w| >= thresh: the smallest
+# float above 0.01 (PYCBF_GPU_REFERENCE_THRESH)
+GPU_REFERENCE_THRESH = 0.010000000707805157
 TXPACK_FLOATS = 16
 RXPACK_FLOATS = 8
 ABI_VERSION = 1
@@ -116,6 +120,8 @@ _SIGS = {
     "pycbf_batch_run": ([_vp, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_batch_autotune": ([_vp, _vp, ct.c_int, _vp, _sz, _vp, _vp, _vp, _vp], ct.c_int),
     "pycbf_batch_latencies": ([_vp, _vp, _sz, _vp], ct.c_int),
+    "pycbf_measure_bandwidth": ([ct.c_int, _vp, _sz, ct.c_int, _vp, _sz,
+                                 ct.POINTER(ct.c_double), _vp], ct.c_int),
     "pycbf_profile_enable": ([ct.c_int], None),
     "pycbf_profile_reset": ([], None),
     "pycbf_profile_read": ([ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)],
@@ -174,3 +180,20 @@ def profile_read(kernel: str):
     n = ct.c_int64(0)
     lib.pycbf_profile_read(kernel.encode(), ct.byref(ms), ct.byref(n))
     return ms.value, n.value
+
+
+def measure_bandwidth(kind: str, device, nbytes: int = 0, reps: int = 5) -> float:
+    """Measured read bandwidth (GB/s) of 'hbm' (a streaming read of nbytes,
+    default 4 GiB), 'l2' (workgroups re-reading a 2 MiB region) or 'lds'."""
+    import torch
+    code = {"hbm": 0, "l2": 1, "lds": 2}[kind]
+    if not nbytes:
+        nbytes = {"hbm": 4 << 30, "l2": 2 << 20, "lds": 16}[kind]
+    buf = torch.ones(nbytes // 4, dtype=torch.float32, device=device)
+    sink = torch.empty(8192 * 256, dtype=torch.int32, device=device)
+    out = ct.c_double(0.0)
+    check(lib.pycbf_measure_bandwidth(code, ptr(buf), buf.numel() * 4, reps, ptr(sink),
+                                      sink.numel() * 4, ct.byref(out), stream_of(device)),
+          "pycbf_measure_bandwidth")
+    return out.value
+
