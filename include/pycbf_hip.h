@@ -136,6 +136,26 @@ int pycbf_das_synthetic(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                         const float *S, float thresh, int sumflag, float *out, void *workspace,
                         size_t workspace_bytes, void *stream);
 
+/* No-allocation variant of pycbf_das_synthetic.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as pycbf_das_synthetic without the workspace:
+ * the library uses its own device array of PYCBF_NOALLOC_SCRATCH_BYTES per GPU
+ * (the calling thread's current device, which the stream must belong to),
+ * held until the process ends, and calls no allocator.  When the outer
+ * split's partial images do not fit, fewer partials are used (the same sums in
+ * another order); a problem whose spline table, tile ranges and tables still
+ * exceed the array returns PYCBF_EINVAL with the size it needs.  Example: one
+ * frame of 31 x 128 channels x 4096 samples, cubic, fits (132 MB); one frame
+ * of 75 x 128 x 4096 cubic does not (315 MB of spline table).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define PYCBF_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int pycbf_das_synthetic_noalloc(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                                const float *txpack, const float *rxpack, float c0,
+                                const float *pvec, const int32_t *perm, int64_t np,
+                                int interp_kind, int interp_param, const float *S, float thresh,
+                                int sumflag, float *out, void *stream);
+
 /* Delay / apodisation tables from the synthetic geometry (the per-sample
  * calc_tautx_apodtx / calc_taurx_apodrx of pycbf/gpu/__engines__.cu:542-561). */
 int pycbf_synthetic_tables(const pycbf_rfinfo *info, const float *txpack, const float *rxpack,

@@ -86,6 +86,9 @@ _SIGS = {
                               _vp, _vp], ct.c_int),
     "pycbf_das_synthetic": ([_prf, _vp, _i64, _vp, _vp, _f32, _vp, _vp, _i64, ct.c_int,
                              ct.c_int, _vp, _f32, ct.c_int, _vp, _vp, _sz, _vp], ct.c_int),
+    "pycbf_das_synthetic_noalloc": ([_prf, _vp, _i64, _vp, _vp, _f32, _vp, _vp, _i64,
+                                     ct.c_int, ct.c_int, _vp, _f32, ct.c_int, _vp, _vp],
+                                    ct.c_int),
     "pycbf_synthetic_tables": ([_prf, _vp, _vp, _f32, _vp, _i64, _vp, _vp, _vp, _vp, _vp],
                                ct.c_int),
     "pycbf_tile_order": ([_vp, _i64, ct.c_int, _vp], ct.c_int),
