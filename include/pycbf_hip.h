@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PYCBF_ABI_VERSION 1
+#define PYCBF_ABI_VERSION 2
 #define PYCBF_EINVAL 1001
 
 /* interp kinds: 0..3 keep pycbf's interp_keys (pycbf/gpu/__core_bf_classes__.py:215,332) */
@@ -112,6 +112,21 @@ int pycbf_das_tabbed(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                      const float *apodrx, const int32_t *perm, int64_t np, int interp_kind,
                      int interp_param, const float *S, float thresh, int sumflag, float *out,
                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* pycbf_das_tabbed with float32 residuals of the delays (no reference
+ * counterpart: for delays computed beyond float32, as pycbf_ray_delays writes
+ * them for the speed-of-sound chain).  Each sample is interpolated at
+ * (tautx + tautx_lo) + (taurx + taurx_lo) summed in double, on the piece
+ * j = floor(q) with exact weights b = q - j, a = 1 - b; the weight threshold
+ * and the trace bounds follow the float32 delays.  Either residual table may
+ * be NULL (zero), not both.  Runs the direct-gather kernel; other arguments
+ * and the output layout as pycbf_das_tabbed. */
+int pycbf_das_tabbed_residual(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                              const float *tautx, const float *apodtx, const float *taurx,
+                              const float *apodrx, const float *tautx_lo, const float *taurx_lo,
+                              int64_t np, int interp_kind, int interp_param, const float *S,
+                              float thresh, int sumflag, float *out, void *workspace,
+                              size_t workspace_bytes, void *stream);
 
 /* HOST function.  Builds the per-element geometry packs consumed on the device
  * from pycbf's Synthetic parameters (pycbf/__bf_base_classes__.py:141-183, all
@@ -260,13 +275,20 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
  * the Hermite coefficient construction is applied to grf), akima or makima
  * (the adjoint through the Akima derivatives, whose weights |.| differentiate
  * to sign(.) with sign(0) = 0).  Samples with |apodtx*apodrx| < thresh
- * contribute nothing, as in the forward. */
+ * contribute nothing, as in the forward.  tautx_lo (ntx, np) and taurx_lo
+ * (nrx, np) are optional (NULL) float32 residuals of the delays (as
+ * pycbf_ray_delays writes them): with them the delay gradients take the
+ * interpolant's derivative at (tautx + tautx_lo) + (taurx + taurx_lo) summed
+ * in double with exact weights, as pycbf_das_tabbed_residual's forward
+ * (sample selection follows the float32 delays; the channel-data and
+ * apodisation gradients are those of pycbf_das_tabbed's float32 evaluation). */
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          const float *tautx, const float *apodtx, const float *taurx,
-                         const float *apodrx, int64_t np, int interp_kind, int param,
-                         const float *S, float thresh, int sumflag, const float *gout,
-                         float *grf, float *gtautx, float *gapodtx, float *gtaurx,
-                         float *gapodrx, void *workspace, size_t workspace_bytes, void *stream);
+                         const float *apodrx, const float *tautx_lo, const float *taurx_lo,
+                         int64_t np, int interp_kind, int param, const float *S, float thresh,
+                         int sumflag, const float *gout, float *grf, float *gtautx,
+                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
+                         size_t workspace_bytes, void *stream);
 
 /* Straight-ray delays through a slowness map (no reference counterpart: the
  * reference's examples/rough_sos_correction.ipynb uses an external eikonal
@@ -278,11 +300,15 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
  * receive elements at (xele, 0) (device (nrx)): taurx = integral of slow from
  * the element to the pixel.  Midpoint rule with nsteps samples per ray.
  * pvec (np, 2) device pixel positions (x, z); tautx (ntx, np), taurx (nrx, np).
+ * Each ray is evaluated in double from the float32 inputs and rounded to
+ * float32; tautx_lo / taurx_lo (same shapes, optional: NULL) receive the
+ * residuals delay - float32(delay) for pycbf_das_tabbed_bwd.
  * The map has at most 8192 nodes. */
 int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                      float dzm, const float *steer, const float *xorig, int ntx, float c0,
                      const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
-                     float *tautx, float *taurx, void *stream);
+                     float *tautx, float *taurx, float *tautx_lo, float *taurx_lo,
+                     void *stream);
 
 /* Workspace bytes of pycbf_ray_delays_bwd (64-bit partial maps of its
  * workgroups). */

@@ -175,7 +175,8 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
                    const float *atx, const float *trx, const float *arx, const int32_t *perm,
                    int64_t np, int kind, int param, const float *S, float thresh, int sumflag,
                    float *out, const DasPlan &plan, Carver &cv, hipStream_t s,
-                   const SynGeo &geo = SynGeo())
+                   const SynGeo &geo = SynGeo(), const float *ttx_lo = nullptr,
+                   const float *trx_lo = nullptr)
 {
     const int nt = info->tinfo.nx;
     const float t0 = info->tinfo.x0, dt = info->tinfo.dx;
@@ -205,6 +206,8 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
     a.rxpack = geo.rxpack;
     a.pvec = geo.pvec;
     a.c0 = geo.c0;
+    a.ttx_lo = ttx_lo;
+    a.trx_lo = trx_lo;
     int rc = 0;
     switch (kind) {
     case PYCBF_INTERP_NEAREST:
@@ -254,13 +257,15 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
         break;
     }
     }
-    if (das_use_plain()) return launch_das_plain(plan.gk, a, s);
-    if (das_reg_applies(plan.gk, nfr, sumflag)) return launch_das_reg(a, s);
     float *part = nullptr;
     if (plan.part) {
         part = (float *)cv.take(plan.part);
         if (!part) return fail(PYCBF_EINVAL, "workspace too small");
     }
+    // delay residuals: the direct gather places samples in double
+    if (ttx_lo || trx_lo) return launch_das(plan.gk, a, plan.nsplit, part, s);
+    if (das_use_plain()) return launch_das_plain(plan.gk, a, s);
+    if (das_reg_applies(plan.gk, nfr, sumflag)) return launch_das_reg(a, s);
     if (plan.bounds) {
         float *bnd = (float *)cv.take(plan.bounds);
         if (!bnd) return fail(PYCBF_EINVAL, "workspace too small");
@@ -380,6 +385,29 @@ int pycbf_das_tabbed(const pycbf_rfinfo *info, const float *rf, int64_t nfr, con
     Carver cv(workspace, workspace_bytes);
     return run_das(info, rf, nfr, tautx, apodtx, taurx, apodrx, perm, np, interp_kind,
                    interp_param, S, thresh, sumflag, out, plan, cv, (hipStream_t)stream);
+}
+
+int pycbf_das_tabbed_residual(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                              const float *tautx, const float *apodtx, const float *taurx,
+                              const float *apodrx, const float *tautx_lo, const float *taurx_lo,
+                              int64_t np, int interp_kind, int interp_param, const float *S,
+                              float thresh, int sumflag, float *out, void *workspace,
+                              size_t workspace_bytes, void *stream)
+{
+    int rc = validate_common(info, rf, nfr, np, interp_kind, interp_param, S, sumflag, out);
+    if (rc) return rc;
+    if (!tautx || !apodtx || !taurx || !apodrx)
+        return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
+    if (!tautx_lo && !taurx_lo)
+        return fail(PYCBF_EINVAL, "give tautx_lo or taurx_lo (else call pycbf_das_tabbed)");
+    const DasPlan plan = plan_das(info, nfr, np, interp_kind, interp_param, 0, sumflag);
+    if (workspace_bytes < plan.total || !workspace)
+        return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
+                    workspace_bytes);
+    Carver cv(workspace, workspace_bytes);
+    return run_das(info, rf, nfr, tautx, apodtx, taurx, apodrx, nullptr, np, interp_kind,
+                   interp_param, S, thresh, sumflag, out, plan, cv, (hipStream_t)stream, SynGeo(),
+                   tautx_lo, taurx_lo);
 }
 
 int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, const double *xele,
@@ -599,10 +627,11 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
 
 int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          const float *tautx, const float *apodtx, const float *taurx,
-                         const float *apodrx, int64_t np, int interp_kind, int param,
-                         const float *S, float thresh, int sumflag, const float *gout,
-                         float *grf, float *gtautx, float *gapodtx, float *gtaurx,
-                         float *gapodrx, void *workspace, size_t workspace_bytes, void *stream)
+                         const float *apodrx, const float *tautx_lo, const float *taurx_lo,
+                         int64_t np, int interp_kind, int param, const float *S, float thresh,
+                         int sumflag, const float *gout, float *grf, float *gtautx,
+                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
+                         size_t workspace_bytes, void *stream)
 {
     if (interp_kind != PYCBF_INTERP_NEAREST && interp_kind != PYCBF_INTERP_LINEAR &&
         interp_kind != PYCBF_INTERP_CUBIC && interp_kind != PYCBF_INTERP_KORDER_CUBIC &&
@@ -635,6 +664,8 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
     b.atx = apodtx;
     b.trx = taurx;
     b.arx = apodrx;
+    b.ttx_lo = tautx_lo;
+    b.trx_lo = taurx_lo;
     b.np = np;
     b.ntx = (int)info->ntx;
     b.nrx = (int)info->nrx;
@@ -751,6 +782,10 @@ static int ray_args(const float *slow, int nzm, int nxm, float xm0, float zm0, f
     a.g.zm0 = zm0;
     a.g.rdxm = 1.0f / dxm;
     a.g.rdzm = 1.0f / dzm;
+    a.xm0d = (double)xm0;
+    a.zm0d = (double)zm0;
+    a.dxmd = (double)dxm;
+    a.dzmd = (double)dzm;
     a.steer = steer;
     a.xorig = xorig;
     a.xele = xele;
@@ -766,7 +801,7 @@ static int ray_args(const float *slow, int nzm, int nxm, float xm0, float zm0, f
 int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, float dxm,
                      float dzm, const float *steer, const float *xorig, int ntx, float c0,
                      const float *xele, int nrx, const float *pvec, int64_t np, int nsteps,
-                     float *tautx, float *taurx, void *stream)
+                     float *tautx, float *taurx, float *tautx_lo, float *taurx_lo, void *stream)
 {
     RayArgs a{};
     if (int rc = ray_args(slow, nzm, nxm, xm0, zm0, dxm, dzm, steer, xorig, ntx, c0, xele, nrx,
@@ -775,6 +810,8 @@ int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, 
     if ((ntx > 0 && !tautx) || (nrx > 0 && !taurx)) return fail(PYCBF_EINVAL, "NULL output");
     a.tautx = tautx;
     a.taurx = taurx;
+    a.tautx_lo = tautx_lo;
+    a.taurx_lo = taurx_lo;
     return launch_ray_delays(a, 0, (hipStream_t)stream);
 }
 

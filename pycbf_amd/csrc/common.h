@@ -87,6 +87,9 @@ struct DasArgs {
     float c0;
     const float *bounds;     // (ntiles, ntx + nrx, 2) delay min/max per pixel tile
     int nent;                // table entries per channel row of src
+    // direct gather only: float32 delay residuals (may be null); samples are
+    // placed at (ttx + ttx_lo) + (trx + trx_lo) in double, exact weights
+    const float *ttx_lo, *trx_lo;
 };
 
 int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
@@ -147,6 +150,9 @@ struct BwdArgs {
     int modified;         // akima (0) / makima (1)
     const float *rf;      // akima / makima: the raw channel data (slopes are recomputed)
     unsigned *mbits;      // 3 words of workspace: max |gout|, |apodtx|, |apodrx| (bits)
+    const float *ttx_lo, *trx_lo;  // delay residuals (may be null): the delay gradient
+                                   // evaluates the interpolant's derivative at
+                                   // (tautx + tautx_lo) + (taurx + taurx_lo) in double
 };
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
                    float *gapodrx, float *grf, hipStream_t s);
@@ -188,7 +194,9 @@ struct RayArgs {
     float c0;
     const float *pvec;  // (np, 2)
     int64_t np;
+    double xm0d, zm0d, dxmd, dzmd;     // map geometry in double (forward)
     float *tautx, *taurx;              // forward outputs
+    float *tautx_lo, *taurx_lo;        // forward residuals (may be null)
     const float *gtautx, *gtaurx;      // adjoint inputs
     float *gslow;                      // adjoint output
     float *gpart;                      // adjoint scratch: 256 B (max word), then the workgroups'

@@ -11,6 +11,8 @@
 // Per-sample semantics (pycbf/cpu/pycbf.c:179-271 with the DAS loop of
 // pycbf/gpu/__engines__.cu:813-864): w = apodtx*apodrx, x = tautx+taurx; the
 // sample contributes w*interp(x) when |w| >= thresh and x lies on the trace.
+// With delay residuals (pycbf_das_tabbed_residual) the interpolant is taken
+// at x + residuals in double with exact weights (gather_q).
 #include <math.h>
 #include <stdlib.h>
 
@@ -51,9 +53,14 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
 #pragma unroll
     for (int f = 0; f < FC; ++f) acc[f] = 0.0f;
 
+    // delay residuals (workgroup-uniform): positions in double, exact weights
+    const bool fine = A.ttx_lo || A.trx_lo;
+    const float *la_t = OUTER_TX ? A.ttx_lo : A.trx_lo;
+    const float *lb_t = OUTER_TX ? A.trx_lo : A.ttx_lo;
     for (int a = a0; a < a1; ++a) {
         const float ta = ta_t[(int64_t)a * np + p];
         const float wa = wa_t[(int64_t)a * np + p];
+        const double la = (fine && la_t) ? (double)la_t[(int64_t)a * np + p] : 0.0;
         for (int b = 0; b < nb; ++b) {
             const float tb = tb_t[(int64_t)b * np + p];
             const float wb = wb_t[(int64_t)b * np + p];
@@ -65,7 +72,14 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
             float v[FC];
             bool ok = false;
             const bool use = fabsf(w) >= A.thresh;
-            if (use) gather<GK, FC>(A, ch0, fstride, nf, x, ok, v);
+            if (use && fine) {
+                const double lb = lb_t ? (double)lb_t[(int64_t)b * np + p] : 0.0;
+                const double q = (((double)ta + (double)tb) + (la + lb) - (double)A.t0) /
+                                 (double)A.dt;
+                gather_q<GK, FC>(A, ch0, fstride, nf, x, q, ok, v);
+            } else if (use) {
+                gather<GK, FC>(A, ch0, fstride, nf, x, ok, v);
+            }
             if (SUM == PYCBF_SUM_NONE) {
                 float *o = A.out + ((((int64_t)f0 * A.ntx + tx) * A.nrx + rx) * np + p);
 #pragma unroll

@@ -99,6 +99,35 @@ __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t
     }
 }
 
+// Derivative of the interpolant at the sample position q = (x - t0) / dt held
+// in double (x placed to sub-float32 precision by delay residuals): the same
+// piece as interp_vd, evaluated at the exact fractional position instead of
+// the float32 rounding of x and of the knot times.
+template <int GK>
+__device__ __forceinline__ float interp_d_at(const float *__restrict__ src, int64_t row, int n,
+                                             double q, float rdt)
+{
+    if (GK == GK_NEAREST) return 0.0f;
+    int j = (int)floor(q);
+    j = min(max(j, 0), n - 1);
+    const double fr = q - (double)j;
+    if (GK == GK_KORDER || GK == GK_AKIMA) {
+        const float t = (float)fr;
+        const float4 c = reinterpret_cast<const float4 *>(src)[row * n + j];
+        return __builtin_fmaf(__builtin_fmaf(3.0f * c.w, t, 2.0f * c.z), t, c.y) * rdt;
+    }
+    if (GK == GK_LINEAR) {
+        const float y0 = src[row * n + j];
+        const float y1 = (j + 1 < n) ? src[row * n + j + 1] : 0.0f;
+        return (y1 - y0) * rdt;
+    }
+    const float b = (float)fr, a = (float)(1.0 - fr);
+    const Pair2 c = *reinterpret_cast<const Pair2 *>(src + (row * (n + 1) + j) * 2);
+    const float da = __builtin_fmaf(3.0f * a, a, -1.0f);
+    const float db = __builtin_fmaf(3.0f * b, b, -1.0f);
+    return ((c.y1 - c.y0) + (db * c.z1 - da * c.z0)) * rdt;
+}
+
 __device__ __forceinline__ int64_t gout_index(int sumflag, int f, int tx, int rx, int ntx, int nrx,
                                               int64_t np, int64_t p)
 {
@@ -126,17 +155,30 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
     const float *ai = OUTER_TX ? A.arx : A.atx;
     const float tau_o = to[(int64_t)o * A.np + p];
     const float ap_o = ao[(int64_t)o * A.np + p];
+    // delay residuals: the derivative is taken at the position in double
+    const float *lo_o_t = OUTER_TX ? A.ttx_lo : A.trx_lo;
+    const float *lo_i_t = OUTER_TX ? A.trx_lo : A.ttx_lo;
+    const bool fine = (A.ttx_lo || A.trx_lo) && GK != GK_NEAREST;
+    const double lo_o = lo_o_t ? (double)lo_o_t[(int64_t)o * A.np + p] : 0.0;
     float gt = 0.0f, ga = 0.0f;
     for (int i = 0; i < ninner; ++i) {
         const float ap_i = ai[(int64_t)i * A.np + p];
         const float w = OUTER_TX ? ap_o * ap_i : ap_i * ap_o;
         if (!(fabsf(w) >= A.thresh)) continue;
-        const float x = OUTER_TX ? tau_o + ti[(int64_t)i * A.np + p] : ti[(int64_t)i * A.np + p] + tau_o;
+        const float tau_i = ti[(int64_t)i * A.np + p];
+        const float x = OUTER_TX ? tau_o + tau_i : tau_i + tau_o;
         const int tx = OUTER_TX ? o : i, rx = OUTER_TX ? i : o;
+        double q = 0.0;
+        if (fine) {
+            const double lo_i = lo_i_t ? (double)lo_i_t[(int64_t)i * A.np + p] : 0.0;
+            const double xd = ((double)tau_o + (double)tau_i) + (lo_o + lo_i);
+            q = (xd - (double)A.t0) / (double)A.dt;
+        }
         for (int f = 0; f < A.nfr; ++f) {
             const int64_t row = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
             float v = 0.0f, d = 0.0f;
             if (!interp_vd<GK>(A.src, row, A.nt, x, A.t0, A.dt, A.rdt, A.tmax, v, d)) break;
+            if (fine) d = interp_d_at<GK>(A.src, row, A.nt, q, A.rdt);
             const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)];
             gt = __builtin_fmaf(g * w, d, gt);
             ga = __builtin_fmaf(g * ap_i, v, ga);

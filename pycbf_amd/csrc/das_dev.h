@@ -210,4 +210,56 @@ __device__ __forceinline__ void gather(const DasArgs &A, int64_t ch0, int64_t fs
     }
 }
 
+// As gather, at the sample position q = (x - t0) / dt held in double (x
+// placed beyond float32 by delay residuals): the piece j = floor(q) with the
+// exact weights b = q - j, a = 1 - b (the reference's separately rounded knot
+// weights differ from these by up to a few 1e-5); the trace bounds follow the
+// float32 x as in gather.  nearest keeps gather's rule.
+template <int GK, int FC>
+__device__ __forceinline__ void gather_q(const DasArgs &A, int64_t ch0, int64_t fstride, int nf,
+                                         float x, double q, bool &ok, float (&v)[FC])
+{
+    if (GK == GK_NEAREST) {
+        gather<GK, FC>(A, ch0, fstride, nf, x, ok, v);
+        return;
+    }
+    ok = x >= A.t0 && x <= A.tmax;
+    if (!ok) return;
+    int j = (int)floor(q);
+    j = min(max(j, 0), A.nsrc - 1);
+    const double fr = q - (double)j;
+    if (GK == GK_LINEAR) {
+        const float b = (float)fr, a = (float)(1.0 - fr);
+        const bool hi = (j + 1) < A.nsrc;
+        const float *s = A.src + ch0 * A.ch_stride + j;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) v[f] = a * s[f * fstride] + b * (hi ? s[f * fstride + 1] : 0.0f);
+    } else if (GK == GK_CUBIC) {
+        const float b = (float)fr, a = (float)(1.0 - fr);
+        const float ca = __builtin_fmaf(a * a, a, -a);
+        const float cb = __builtin_fmaf(b * b, b, -b);
+        const float *s = A.src + ch0 * A.ch_stride + 2 * j;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) {
+                const Pair2 c = *reinterpret_cast<const Pair2 *>(s + f * fstride);
+                float r = a * c.y0;
+                r = __builtin_fmaf(b, c.y1, r);
+                r = __builtin_fmaf(ca, c.z0, r);
+                v[f] = __builtin_fmaf(cb, c.z1, r);
+            }
+    } else {
+        const float t = (float)fr;
+        const float4 *s = reinterpret_cast<const float4 *>(A.src) + ch0 * (A.ch_stride >> 2) + j;
+        const int64_t fs4 = fstride >> 2;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) {
+                const float4 c = s[f * fs4];
+                v[f] = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, t, c.z), t, c.y), t, c.x);
+            }
+    }
+}
+
 }  // namespace pycbf

@@ -8,7 +8,11 @@
 //     along (sin theta, cos theta) enters z = 0 at q = (x_p - z_p tan theta, 0);
 //     tau_tx = (q_x - xo) sin theta / c0 + integral of s from q to p
 //   receive element at (x_e, 0): tau_rx = integral of s from (x_e, 0) to p
-// The integral is the midpoint rule (summed in double) with nsteps samples of the bilinear
+// The forward evaluates every ray in double (sample positions, bilinear weights,
+// path length and the fire term from the float32 inputs) and rounds the delay
+// to float32; it can also write the float32 residual delay - float32(delay),
+// which the DAS adjoint uses to place each sample to sub-float32 precision.
+// The integral is the midpoint rule with nsteps samples of the bilinear
 // interpolant of the slowness map s (nzm, nxm) whose node (i, k) sits at
 // (xm0 + (k + 1/2) dxm, zm0 + (i + 1/2) dzm); outside the node range the map
 // is extended by its edge values.  The adjoint scatters
@@ -62,6 +66,38 @@ __device__ __forceinline__ void ray_of(int e, int ntx, const float *steer, const
 
 }  // namespace
 
+// the same in double, from the float32 inputs and the map geometry in double
+__device__ __forceinline__ void bilin_d(const RayArgs &a, double x, double z, int &i0, int &k0,
+                                        double &wz, double &wx)
+{
+    const int nxm = a.g.nxm, nzm = a.g.nzm;
+    double u = (x - a.xm0d) / a.dxmd - 0.5;
+    double v = (z - a.zm0d) / a.dzmd - 0.5;
+    u = fmin(fmax(u, 0.0), (double)(nxm - 1));
+    v = fmin(fmax(v, 0.0), (double)(nzm - 1));
+    k0 = min((int)u, max(nxm - 2, 0));
+    i0 = min((int)v, max(nzm - 2, 0));
+    wx = nxm > 1 ? u - (double)k0 : 0.0;
+    wz = nzm > 1 ? v - (double)i0 : 0.0;
+}
+
+__device__ __forceinline__ void ray_of_d(const RayArgs &a, int e, double px, double pz,
+                                         double &qx, double &L, double &fire)
+{
+    if (e < a.ntx) {
+        const double th = (double)a.steer[e];
+        const double sn = sin(th), cs = cos(th);
+        qx = px - pz * (sn / cs);
+        L = pz / cs;
+        fire = (qx - (double)a.xorig[e]) * sn / (double)a.c0;
+    } else {
+        qx = (double)a.xele[e - a.ntx];
+        const double dx = px - qx;
+        L = sqrt(dx * dx + pz * pz);
+        fire = 0.0;
+    }
+}
+
 __global__ void k_fill_u32(unsigned *p, int n, unsigned v)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = v;
@@ -77,28 +113,33 @@ __global__ __launch_bounds__(256) void k_ray_delays(RayArgs a)
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int e = blockIdx.y;
     if (p >= a.np) return;
-    const float px = a.pvec[2 * p], pz = a.pvec[2 * p + 1];
-    float qx, L, fire;
-    ray_of(e, a.ntx, a.steer, a.xorig, a.xele, a.c0, px, pz, qx, L, fire);
-    const float rn = 1.0f / (float)a.nsteps;
-    // the ray sum runs in double: a float32 sum of nsteps samples of ~1/c
-    // would move the delay by a few thousandths of a sample
+    const double px = a.pvec[2 * p], pz = a.pvec[2 * p + 1];
+    double qx, L, fire;
+    ray_of_d(a, e, px, pz, qx, L, fire);
+    const int nxm = a.g.nxm, nzm = a.g.nzm;
     double acc = 0.0;
     for (int m = 0; m < a.nsteps; ++m) {
-        const float t = ((float)m + 0.5f) * rn;
-        const float x = qx + t * (px - qx), z = t * pz;
+        const double t = ((double)m + 0.5) / (double)a.nsteps;
+        const double x = qx + t * (px - qx), z = t * pz;
         int i0, k0;
-        float wz, wx;
-        bilin(a.g, x, z, i0, k0, wz, wx);
-        const int i1 = min(i0 + 1, a.g.nzm - 1), k1 = min(k0 + 1, a.g.nxm - 1);
-        const float s00 = sm[i0 * a.g.nxm + k0], s01 = sm[i0 * a.g.nxm + k1];
-        const float s10 = sm[i1 * a.g.nxm + k0], s11 = sm[i1 * a.g.nxm + k1];
-        const float top = s00 + wx * (s01 - s00), bot = s10 + wx * (s11 - s10);
-        acc += (double)(top + wz * (bot - top));
+        double wz, wx;
+        bilin_d(a, x, z, i0, k0, wz, wx);
+        const int i1 = min(i0 + 1, nzm - 1), k1 = min(k0 + 1, nxm - 1);
+        const double s00 = sm[i0 * nxm + k0], s01 = sm[i0 * nxm + k1];
+        const double s10 = sm[i1 * nxm + k0], s11 = sm[i1 * nxm + k1];
+        const double top = s00 + wx * (s01 - s00), bot = s10 + wx * (s11 - s10);
+        acc += top + wz * (bot - top);
     }
-    const float tau = (float)((double)fire + (double)L * (acc / (double)a.nsteps));
-    if (e < a.ntx) a.tautx[(int64_t)e * a.np + p] = tau;
-    else a.taurx[(int64_t)(e - a.ntx) * a.np + p] = tau;
+    const double tau = fire + L * (acc / (double)a.nsteps);
+    const float hi = (float)tau;
+    const float lo = (float)(tau - (double)hi);
+    if (e < a.ntx) {
+        a.tautx[(int64_t)e * a.np + p] = hi;
+        if (a.tautx_lo) a.tautx_lo[(int64_t)e * a.np + p] = lo;
+    } else {
+        a.taurx[(int64_t)(e - a.ntx) * a.np + p] = hi;
+        if (a.taurx_lo) a.taurx_lo[(int64_t)(e - a.ntx) * a.np + p] = lo;
+    }
 }
 
 // Adjoint accumulation in 64-bit fixed point.  Float atomic adds in LDS run

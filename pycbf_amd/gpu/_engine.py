@@ -41,7 +41,7 @@ SLSC_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 GPU_REFERENCE_THRESH = 0.010000000707805157
 TXPACK_FLOATS = 16
 RXPACK_FLOATS = 8
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class xInfo(ct.Structure):
@@ -82,6 +82,9 @@ _SIGS = {
     "pycbf_das_workspace_bytes": ([_prf, _i64, _i64, ct.c_int, ct.c_int, ct.c_int], _sz),
     "pycbf_das_tabbed": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, ct.c_int, ct.c_int,
                           _vp, _f32, ct.c_int, _vp, _vp, _sz, _vp], ct.c_int),
+    "pycbf_das_tabbed_residual": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                   ct.c_int, ct.c_int, _vp, _f32, ct.c_int, _vp, _vp, _sz, _vp],
+                                  ct.c_int),
     "pycbf_synthetic_pack": ([ct.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                               _vp, _vp], ct.c_int),
     "pycbf_das_synthetic": ([_prf, _vp, _i64, _vp, _vp, _f32, _vp, _vp, _i64, ct.c_int,
@@ -107,11 +110,12 @@ _SIGS = {
     "pycbf_slsc_selection": ([_vp, _i64, _i64, _i64, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                               _vp, _vp], ct.c_int),
     "pycbf_das_bwd_workspace_bytes": ([_prf, _i64, _i64, ct.c_int, ct.c_int], _sz),
-    "pycbf_das_tabbed_bwd": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _i64, ct.c_int, ct.c_int,
-                              _vp, _f32, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp],
-                             ct.c_int),
+    "pycbf_das_tabbed_bwd": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ct.c_int,
+                              ct.c_int, _vp, _f32, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                              _sz, _vp], ct.c_int),
     "pycbf_ray_delays": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp, ct.c_int,
-                          _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp], ct.c_int),
+                          _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp, _vp, _vp],
+                         ct.c_int),
     "pycbf_ray_delays_bwd_workspace_bytes": ([ct.c_int, ct.c_int], _sz),
     "pycbf_ray_delays_bwd": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp,
                               ct.c_int, _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp,

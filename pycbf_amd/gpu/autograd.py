@@ -76,7 +76,7 @@ def _das_function():
 
     class DASFunction(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, rf, tautx, apodtx, taurx, apodrx, cfg):
+        def forward(ctx, rf, tautx, apodtx, taurx, apodrx, cfg, tautx_lo=None, taurx_lo=None):
             nfr, ntx, nrx, nt = rf.shape
             np_ = tautx.shape[1]
             info = E.rfinfo(ntx, nrx, 0, cfg.t0, cfg.dt, nt)
@@ -86,14 +86,23 @@ def _das_function():
             out = torch.empty(_out_shape(cfg.sumtype, nfr, ntx, nrx, np_), dtype=torch.float32,
                               device=rf.device)
             S = cfg.S_on(rf.device)
-            rc = E.lib.pycbf_das_tabbed(ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx),
-                                        E.ptr(apodtx), E.ptr(taurx), E.ptr(apodrx), None, np_,
-                                        kind, cfg.param, E.ptr(S), cfg.thresh,
-                                        E.SUMTYPE[cfg.sumtype], E.ptr(out), E.ptr(ws),
-                                        ws.numel(), E.stream_of(rf.device))
-            E.check(rc, "pycbf_das_tabbed")
+            if tautx_lo is None and taurx_lo is None:
+                rc = E.lib.pycbf_das_tabbed(ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx),
+                                            E.ptr(apodtx), E.ptr(taurx), E.ptr(apodrx), None, np_,
+                                            kind, cfg.param, E.ptr(S), cfg.thresh,
+                                            E.SUMTYPE[cfg.sumtype], E.ptr(out), E.ptr(ws),
+                                            ws.numel(), E.stream_of(rf.device))
+                E.check(rc, "pycbf_das_tabbed")
+            else:
+                rc = E.lib.pycbf_das_tabbed_residual(
+                    ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx), E.ptr(apodtx),
+                    E.ptr(taurx), E.ptr(apodrx), E.ptr(tautx_lo), E.ptr(taurx_lo), np_, kind,
+                    cfg.param, E.ptr(S), cfg.thresh, E.SUMTYPE[cfg.sumtype], E.ptr(out),
+                    E.ptr(ws), ws.numel(), E.stream_of(rf.device))
+                E.check(rc, "pycbf_das_tabbed_residual")
             ctx.save_for_backward(rf, tautx, apodtx, taurx, apodrx)
             ctx.cfg = cfg
+            ctx.lo = (tautx_lo, taurx_lo)
             return out
 
         @staticmethod
@@ -112,14 +121,16 @@ def _das_function():
             need = E.lib.pycbf_das_bwd_workspace_bytes(ctypes.byref(info), nfr, np_, kind,
                                                        cfg.param)
             ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
+            ttx_lo, trx_lo = ctx.lo
             rc = E.lib.pycbf_das_tabbed_bwd(
                 ctypes.byref(info), E.ptr(rf), nfr, E.ptr(tautx), E.ptr(apodtx), E.ptr(taurx),
-                E.ptr(apodrx), np_, kind, cfg.param, E.ptr(cfg.S_on(rf.device)), cfg.thresh,
+                E.ptr(apodrx), E.ptr(ttx_lo), E.ptr(trx_lo), np_, kind, cfg.param,
+                E.ptr(cfg.S_on(rf.device)), cfg.thresh,
                 E.SUMTYPE[cfg.sumtype], E.ptr(gout),
                 E.ptr(grf), E.ptr(gttx), E.ptr(gatx), E.ptr(gtrx), E.ptr(garx), E.ptr(ws),
                 ws.numel(), E.stream_of(rf.device))
             E.check(rc, "pycbf_das_tabbed_bwd")
-            return grf, gttx, gatx, gtrx, garx, None
+            return grf, gttx, gatx, gtrx, garx, None, None, None
 
     return DASFunction
 
@@ -128,7 +139,7 @@ _FN = None
 
 
 def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype="tx_and_rx",
-               thresh=0.0, k=None, usf=1):
+               thresh=0.0, k=None, usf=1, tautx_lo=None, taurx_lo=None):
     """Differentiable Tabbed DAS on the GPU.
 
     rf (ntx, nrx, nt) or (nfr, ntx, nrx, nt); tautx / apodtx (ntx, np);
@@ -138,7 +149,11 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype
     'nearest' takes ``usf`` (the spline upsampling factor, default 1).  The
     output has the layout of TabbedBeamformer for ``sumtype``.
     ``thresh`` (default 0: every sample) drops samples with
-    |apodtx * apodrx| < thresh in the forward and the adjoint alike."""
+    |apodtx * apodrx| < thresh in the forward and the adjoint alike.
+    ``tautx_lo`` / ``taurx_lo`` (optional, constant) are float32 residuals of
+    the delays (``ray_delays(..., residuals=True)``): the forward
+    (pycbf_das_tabbed_residual) and the delay gradients then place each
+    sample at the delays summed in double with their residuals."""
     global _FN
     if _FN is None:
         _FN = _das_function()
@@ -154,7 +169,11 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype
     apodtx = _check(apodtx, "apodtx", (ntx, np_))
     taurx = _check(taurx, "taurx", (nrx, np_))
     apodrx = _check(apodrx, "apodrx", (nrx, np_))
-    out = _FN.apply(rf4, tautx, apodtx, taurx, apodrx, cfg)
+    if tautx_lo is not None:
+        tautx_lo = _check(tautx_lo.detach(), "tautx_lo", (ntx, np_))
+    if taurx_lo is not None:
+        taurx_lo = _check(taurx_lo.detach(), "taurx_lo", (nrx, np_))
+    out = _FN.apply(rf4, tautx, apodtx, taurx, apodrx, cfg, tautx_lo, taurx_lo)
     return out[0] if squeeze else out
 
 

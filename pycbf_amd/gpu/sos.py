@@ -56,19 +56,21 @@ def _ray_function():
         @staticmethod
         def forward(ctx, slow, geo, steer, xorig, xele, pvec, c0, nsteps):
             ntx, nrx, np_ = steer.numel(), xele.numel(), pvec.shape[0]
-            tautx = torch.empty((ntx, np_), dtype=torch.float32, device=slow.device)
-            taurx = torch.empty((nrx, np_), dtype=torch.float32, device=slow.device)
+            mk = lambda n: torch.empty((n, np_), dtype=torch.float32, device=slow.device)  # noqa: E731
+            tautx, taurx, tautx_lo, taurx_lo = mk(ntx), mk(nrx), mk(ntx), mk(nrx)
             rc = E.lib.pycbf_ray_delays(E.ptr(slow), geo.nzm, geo.nxm, geo.xm0, geo.zm0, geo.dxm,
                                         geo.dzm, E.ptr(steer), E.ptr(xorig), ntx, float(c0),
                                         E.ptr(xele), nrx, E.ptr(pvec), np_, int(nsteps),
-                                        E.ptr(tautx), E.ptr(taurx), E.stream_of(slow.device))
+                                        E.ptr(tautx), E.ptr(taurx), E.ptr(tautx_lo),
+                                        E.ptr(taurx_lo), E.stream_of(slow.device))
             E.check(rc, "pycbf_ray_delays")
             ctx.save_for_backward(slow, steer, xorig, xele, pvec)
             ctx.geo, ctx.c0, ctx.nsteps = geo, float(c0), int(nsteps)
-            return tautx, taurx
+            ctx.mark_non_differentiable(tautx_lo, taurx_lo)
+            return tautx, taurx, tautx_lo, taurx_lo
 
         @staticmethod
-        def backward(ctx, gtautx, gtaurx):
+        def backward(ctx, gtautx, gtaurx, _glo_tx=None, _glo_rx=None):
             slow, steer, xorig, xele, pvec = ctx.saved_tensors
             geo = ctx.geo
             gslow = torch.empty_like(slow)
@@ -91,10 +93,13 @@ def _ray_function():
 _RAY = None
 
 
-def ray_delays(slow, geo: SlownessGrid, steer, xorig, xele, pvec, c0=1540.0, nsteps=64):
+def ray_delays(slow, geo: SlownessGrid, steer, xorig, xele, pvec, c0=1540.0, nsteps=64,
+               residuals=False):
     """Straight-ray (tautx (ntx, np), taurx (nrx, np)) through the slowness
     map ``slow`` (nzm, nxm) float32 CUDA tensor (differentiable); steer /
-    xorig (ntx), xele (nrx), pvec (np, 2) float32 CUDA tensors."""
+    xorig (ntx), xele (nrx), pvec (np, 2) float32 CUDA tensors.  With
+    ``residuals`` also the float32 residuals (tautx_lo, taurx_lo) of the
+    delays, evaluated in double (constants for das_tabbed's adjoint)."""
     global _RAY
     torch = _torch()
     if _RAY is None:
@@ -105,8 +110,9 @@ def ray_delays(slow, geo: SlownessGrid, steer, xorig, xele, pvec, c0=1540.0, nst
             raise BeamformerException(f"{name} must be a float32 CUDA tensor")
     if tuple(slow.shape) != (geo.nzm, geo.nxm):
         raise BeamformerException(f"slow must be {(geo.nzm, geo.nxm)}")
-    return _RAY.apply(slow.contiguous(), geo, steer.contiguous(), xorig.contiguous(),
-                      xele.contiguous(), pvec.contiguous(), c0, nsteps)
+    out = _RAY.apply(slow.contiguous(), geo, steer.contiguous(), xorig.contiguous(),
+                     xele.contiguous(), pvec.contiguous(), c0, nsteps)
+    return out if residuals else out[:2]
 
 
 def lag1_coherence(imrx, kwin=5):
@@ -163,12 +169,12 @@ class SoSProblem:
         """Per-receive-channel images (nrx, nx, nz) for the sound-speed map
         cmap (nzm, nxm) in m/s (differentiable)."""
         slow = 1.0 / cmap
-        tautx, taurx = ray_delays(slow, self.geo, self.steer, self.xorig, self.xele, self.pvec,
-                                  self.c0, self.nsteps)
+        tautx, taurx, tlo, rlo = ray_delays(slow, self.geo, self.steer, self.xorig, self.xele,
+                                            self.pvec, self.c0, self.nsteps, residuals=True)
         ones_tx = self._ones(tautx)
         ones_rx = self._ones(taurx)
         imrx = das_tabbed(self.rf, tautx, ones_tx, taurx, ones_rx, self.t0, self.dt, self.interp,
-                          "tx_only", 0.0)
+                          "tx_only", 0.0, tautx_lo=tlo, taurx_lo=rlo)
         nx, nz = self.gridshape
         return imrx.reshape(-1, nx, nz)
 

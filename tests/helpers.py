@@ -52,3 +52,18 @@ def small_synthetic(seed=0, ntx=8, nrx=32, nt=1024, nx=32, nz=48, mode="pw"):
     xs, zs, amp = sim.scatterers(10, 40, (xlim[0] * 0.8, xlim[1] * 0.8), (3e-3, 17e-3), seed)
     rf = sim.simulate_rf(steers, xele, nt, xs, zs, amp, seed=seed)
     return dict(rf=rf, params=params, t0=0.0, dt=1.0 / sim.FS, nt=nt)
+
+
+def float32_inputs(t0, dt, geo=None):
+    """The time axis (and slowness-map geometry) as the kernels receive them:
+    rounded to float32 (pycbf's RFInfo carries t0 and dt as float32).  FP64
+    references of the kernels' functions take these values, so that both
+    sides evaluate the same function."""
+    import dataclasses
+    t32, dt32 = float(np.float32(t0)), float(np.float32(dt))
+    if geo is None:
+        return t32, dt32
+    g32 = dataclasses.replace(geo, **{k: float(np.float32(getattr(geo, k)))
+                                      for k in ("xm0", "zm0", "dxm", "dzm")})
+    return t32, dt32, g32
+

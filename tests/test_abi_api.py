@@ -132,7 +132,8 @@ def test_adjoint_entry_point_validates_before_device_work():
     p = E.ptr(buf)
 
     def call(kind, param=1, S=None, ws=0):
-        return E.lib.pycbf_das_tabbed_bwd(ct.byref(info), p, 1, p, p, p, p, 16, kind, param,
+        return E.lib.pycbf_das_tabbed_bwd(ct.byref(info), p, 1, p, p, p, p, None, None, 16, kind,
+                                          param,
                                           S, 0.0, E.SUMTYPE["tx_and_rx"], p, p, None, None,
                                           None, None, p if ws else None, ws, None)
 

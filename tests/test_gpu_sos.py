@@ -11,6 +11,8 @@ near zero)."""
 import numpy as np
 import pytest
 
+from tests.helpers import float32_inputs
+
 pytestmark = pytest.mark.gpu
 
 
@@ -49,6 +51,59 @@ def test_ray_delays_match_fp64():
     for a, b in ((ttx, rtx), (trx, rrx)):
         err = float((a.double() - b).abs().max() / b.abs().max())
         assert err <= 1e-6, err
+
+
+def test_ray_delay_residuals_place_the_delay_beyond_float32():
+    """Delay + residual (pycbf_ray_delays' float32 residual outputs) agree with
+    the FP64 restatement far below float32 rounding of the delay."""
+    import torch
+    from oracle import diff64
+    from pycbf_amd.gpu.sos import ray_delays
+    prob, ctrue = _problem()
+    slow = 1.0 / torch.as_tensor(ctrue, device="cuda")
+    ttx, trx, tlo, rlo = ray_delays(slow, prob.geo, prob.steer, prob.xorig, prob.xele, prob.pvec,
+                                    1540.0, 64, residuals=True)
+    rtx, rrx = diff64.ray_delays(slow.double(), prob.geo, prob.steer.double(), prob.xorig.double(),
+                                 prob.xele.double(), prob.pvec.double(), 1540.0, 64)
+    for hi, lo, ref in ((ttx, tlo, rtx), (trx, rlo, rrx)):
+        e_hi = float((hi.double() - ref).abs().max() / ref.abs().max())
+        e_lo = float((hi.double() + lo.double() - ref).abs().max() / ref.abs().max())
+        print(f"ray delays: float32 {e_hi:.2e}, with residual {e_lo:.2e} (relative)", flush=True)
+        assert e_lo <= 1e-8 and e_lo < 0.25 * e_hi
+
+
+def test_residual_das_and_delay_gradients_match_fp64():
+    """pycbf_das_tabbed_residual (samples at delay + residual in double) and the
+    delay gradients of pycbf_das_tabbed_bwd with residuals against the FP64
+    restatement evaluated at the same delays in float64 (x32=False)."""
+    import torch
+    from oracle import diff64
+    from pycbf_amd.gpu.autograd import das_tabbed
+    from pycbf_amd.gpu.sos import ray_delays
+    prob, ctrue = _problem()
+    slow = 1.0 / torch.as_tensor(ctrue, device="cuda")
+    ttx, trx, tlo, rlo = ray_delays(slow, prob.geo, prob.steer, prob.xorig, prob.xele, prob.pvec,
+                                    1540.0, 64, residuals=True)
+    ttx, trx = ttx.clone().requires_grad_(), trx.clone().requires_grad_()
+    ones_t, ones_r = torch.ones_like(ttx), torch.ones_like(trx)
+    im = das_tabbed(prob.rf, ttx, ones_t, trx, ones_r, prob.t0, prob.dt, "cubic", "tx_only", 0.0,
+                    tautx_lo=tlo, taurx_lo=rlo)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8)
+    G = torch.randn(im.shape, generator=g, device="cuda")
+    gt, gr = torch.autograd.grad((im * G).sum(), [ttx, trx])
+    t32, dt32 = float32_inputs(prob.t0, prob.dt)
+    t64 = (ttx.detach().double() + tlo.double()).requires_grad_()
+    r64 = (trx.detach().double() + rlo.double()).requires_grad_()
+    im64 = diff64.das_tabbed(prob.rf.double()[None], t64, ones_t.double(), r64, ones_r.double(),
+                             t32, dt32, "cubic", "tx_only", 0.0, x32=False)[0]
+    e_im = float((im.double() - im64).abs().max() / im64.abs().max())
+    gt64, gr64 = torch.autograd.grad((im64 * G.double()).sum(), [t64, r64])
+    e_t = float((gt.double() - gt64).abs().max() / gt64.abs().max())
+    e_r = float((gr.double() - gr64).abs().max() / gr64.abs().max())
+    print(f"residual DAS vs FP64: image {e_im:.2e}, tx delay grad {e_t:.2e}, rx {e_r:.2e}",
+          flush=True)
+    assert e_im <= 2e-6 and e_t <= 1e-4 and e_r <= 1e-4
 
 
 def test_ray_adjoint_matches_fp64_autograd_and_transpose():
@@ -96,15 +151,21 @@ def test_sos_gradient_matches_fp64_central_differences():
     rf64 = prob.rf.double()[None]
     st, xo, xe, pv = (t.double() for t in (prob.steer, prob.xorig, prob.xele, prob.pvec))
     nx, nz = prob.gridshape
+    t32, dt32, geo32 = float32_inputs(prob.t0, prob.dt, prob.geo)
 
     def L64(cm):
         with torch.no_grad():
-            ttx, trx = diff64.ray_delays(1.0 / cm, prob.geo, st, xo, xe, pv, 1540.0, 64)
+            ttx, trx = diff64.ray_delays(s32 + (1.0 / cm - 1.0 / base), geo32, st, xo, xe, pv,
+                                         1540.0, 64)
             im = diff64.das_tabbed(rf64, ttx, torch.ones_like(ttx), trx, torch.ones_like(trx),
-                                   prob.t0, prob.dt, "cubic", "tx_only", 0.0, x32=False)
+                                   t32, dt32, "cubic", "tx_only", 0.0, x32=False)
             return float(focus_loss(im[0].reshape(-1, nx, nz), e0, prob.kwin))
 
     base = c0map.double()
+    # the kernels take the slowness as float32(1 / c); central differences of
+    # h = 1e-3 m/s lie far below float32 resolution, so the FP64 pipeline
+    # expands around the kernels' base slowness
+    s32 = (1.0 / c0map).double()
     rng = np.random.default_rng(5)
     h = 1e-3
     peak = float(np.abs(gc).max())
