@@ -351,9 +351,11 @@ def sos_leg(dev, steps):
     """Differentiable workload: 31 plane waves over +-15 deg x 128 rx x 4096
     samples, 256 x 256 pixels, a 64 x 64 sound-speed map (1450 m/s above
     10 mm, 1540 m/s below, +-20 m/s blobs); channel data simulated through the
-    true map with the straight-ray model; `steps` Adam steps on the map from a
-    uniform 1540 m/s start, each a forward and a backward pass through the
-    ray-delay and DAS kernels.  Returns the JSON object of the leg."""
+    true map with the straight-ray model; `steps` Adam steps on the 64 x 64
+    map from a uniform 1540 m/s start (lr 2 m/s, a light smoothness penalty),
+    each a forward and a backward pass through the ray-delay and DAS kernels;
+    then the same from a 4 x 4 grid of speeds bilinearly upsampled to the map
+    (a labelled variant).  Returns the JSON object of the leg."""
     import torch
 
     from pycbf_amd import sim
@@ -370,66 +372,85 @@ def sos_leg(dev, steps):
     prob = SoSProblem(rf=rf, steer=f(w["steers"]), xorig=f(w["xorig"]), xele=f(w["xele"]),
                       pvec=f(w["pnts"]), gridshape=w["gridshape"], geo=geo, t0=0.0,
                       dt=1.0 / sim.FS)
-    # the map is parametrised by a coarse 4 x 4 grid of speeds, bilinearly
-    # upsampled to the 64 x 64 map the kernels differentiate (a free 64 x 64
-    # map can raise the focus metric with non-physical maps)
-    theta = torch.full((1, 1, 4, 4), 1540.0, device=dev, requires_grad=True)
-
-    def cmap():
-        return torch.nn.functional.interpolate(theta, size=(w["nzm"], w["nxm"]), mode="bilinear",
-                                               align_corners=True)[0, 0]
-
+    ct = f(ctrue)
+    c0 = torch.full_like(ct, 1540.0)
     with torch.no_grad():
-        e0 = float((prob.images(cmap()).sum(0) ** 2).mean())
-        ct = f(ctrue)
+        e0 = float((prob.images(c0).sum(0) ** 2).mean())
         loss_true = float(prob.loss(ct, e0))
-    opt = torch.optim.Adam([theta], lr=10.0)
     top = int(round((10e-3 - geo.zm0) / geo.dzm))      # map rows above the 10 mm layer boundary
-    rmse0 = float(torch.sqrt(((cmap().detach() - ct) ** 2).mean()))
-    top0 = float(cmap().detach()[:top].mean())
-    losses = []
-    E.lib.pycbf_profile_reset()
-    E.lib.pycbf_profile_enable(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        opt.zero_grad()
-        loss = prob.loss(cmap(), e0)
-        loss.backward()
-        opt.step()
-        # kept on the device: reading it here would stall the host every step
-        losses.append(loss.detach())
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    losses = [float(v) for v in torch.stack(losses).cpu()]
-    E.lib.pycbf_profile_enable(0)
+
+    def optimise(theta, cmap, lr, smooth, timed):
+        prob.smooth_weight = smooth
+        opt = torch.optim.Adam([theta], lr=lr)
+        hist = []
+        if timed:
+            E.lib.pycbf_profile_reset()
+            E.lib.pycbf_profile_enable(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            opt.zero_grad()
+            c = cmap()
+            loss = prob.loss(c, e0)
+            loss.backward()
+            opt.step()
+            # kept on the device: reading here would stall the host every step
+            hist.append(torch.stack([loss.detach().float(),
+                                     torch.sqrt(((c.detach() - ct) ** 2).mean())]))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if timed:
+            E.lib.pycbf_profile_enable(0)
+        prob.smooth_weight = 0.0
+        cfin = cmap().detach()
+        with torch.no_grad():
+            lfin = float(prob.loss(cfin, e0))
+        h = torch.stack(hist).cpu().numpy()
+        return wall, h, cfin, lfin
+
+    free = c0.clone().requires_grad_()
+    wall, hist, cfin, lfin = optimise(free, lambda: free, 2.0, 1e-4, True)
     kern = {}
     for k in ("ray_delays", "ray_delays_bwd", "das_gather", "das_bwd_tau", "spline_coefs"):
         ms, n = E.profile_read(k)
         kern[k] = round(ms / max(n, 1), 4)
-    cfin = cmap().detach()
+    rmse0 = float(torch.sqrt(((c0 - ct) ** 2).mean()))
     rmse1 = float(torch.sqrt(((cfin - ct) ** 2).mean()))
     top1, toptrue = float(cfin[:top].mean()), float(ct[:top].mean())
+    theta = torch.full((1, 1, 4, 4), 1540.0, device=dev, requires_grad=True)
+
+    def coarse():
+        return torch.nn.functional.interpolate(theta, size=(w["nzm"], w["nxm"]), mode="bilinear",
+                                               align_corners=True)[0, 0]
+
+    _, hist4, c4, l4 = optimise(theta, coarse, 10.0, 0.0, False)
     ntx, nrx = len(w["steers"]), len(w["xele"])
     samples = ntx * nrx * w["gridshape"][0] * w["gridshape"][1]
-    log(f"sos leg: {steps} Adam steps {wall * 1e3 / steps:.2f} ms/step, loss {losses[0]:.5f} -> "
-        f"{losses[-1]:.5f} (true map {loss_true:.5f}), map rmse {rmse0:.2f} -> {rmse1:.2f} m/s, "
-        f"top layer {top0:.1f} -> {top1:.1f} (true {toptrue:.1f}) m/s")
+    log(f"sos leg: {steps} Adam steps on the 64x64 map {wall * 1e3 / steps:.2f} ms/step, loss "
+        f"{hist[0, 0]:.5f} -> {lfin:.5f} (true map {loss_true:.5f}), map rmse {rmse0:.2f} -> "
+        f"{rmse1:.2f} m/s, top layer 1540 -> {top1:.1f} (true {toptrue:.1f}) m/s; 4x4 variant "
+        f"loss -> {l4:.5f}, rmse -> {float(torch.sqrt(((c4 - ct) ** 2).mean())):.2f}")
     del rf, prob
     torch.cuda.empty_cache()
     return {
         "workload": f"differentiable DAS for speed of sound: {ntx} PW x {nrx} rx x {w['nt']} "
                     f"samples, {w['gridshape'][0]}x{w['gridshape'][1]} px, "
-                    f"{w['nzm']}x{w['nxm']} map (4x4 speed parameters), straight-ray delays, cubic, "
-                    f"{steps} Adam steps",
+                    f"{w['nzm']}x{w['nxm']} map optimised directly (Adam lr 2 m/s, smoothness "
+                    f"1e-4), straight-ray delays with float32 residuals, cubic, {steps} Adam steps",
         "ms_per_step": round(wall * 1e3 / steps, 3),
         "fwd_bwd_gsamples_per_s": round(samples / (wall / steps) / 1e9, 2),
-        "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
-        "loss_at_true_map": round(loss_true, 6),
+        "loss_per_step": [round(float(v), 5) for v in hist[:, 0]],
+        "map_rmse_per_step_m_per_s": [round(float(v), 3) for v in hist[:, 1]],
+        "loss_final": round(lfin, 6), "loss_at_true_map": round(loss_true, 6),
+        "loss_below_true_map": bool(lfin < loss_true),
         "map_rmse_m_per_s": [round(rmse0, 3), round(rmse1, 3)],
-        "top_layer_mean_m_per_s": {"start": round(top0, 2), "end": round(top1, 2),
+        "top_layer_mean_m_per_s": {"start": 1540.0, "end": round(top1, 2),
                                    "true": round(toptrue, 2)},
         "kernel_ms": kern,
+        "variant_4x4_grid": {"lr": 10.0, "loss_final": round(l4, 6),
+                             "map_rmse_m_per_s": [round(rmse0, 3),
+                                                  round(float(torch.sqrt(((c4 - ct) ** 2).mean())), 3)],
+                             "loss_per_step": [round(float(v), 5) for v in hist4[::5, 0]]},
     }
 
 

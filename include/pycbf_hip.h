@@ -398,9 +398,9 @@ int pycbf_batch_configure(pycbf_batch *batch, int tile, int rchunk, void *worksp
 int pycbf_batch_run(pycbf_batch *batch, const float *rf_all, void *workspace,
                     size_t workspace_bytes, void *stream);
 /* Times the persistent gather for every tile (8, 16, 32) and receive chunk
- * (32, 64, 128) (reps runs each, after one full run that builds the tables),
- * configures the fastest and returns it; times_ms (9 entries, tile-major) may
- * be NULL.  Synchronises. */
+ * (16, 32, 64, 128) (reps runs each, after one full run that builds the
+ * tables), configures the fastest and returns it; times_ms (12 entries,
+ * tile-major) may be NULL.  Synchronises. */
 int pycbf_batch_autotune(pycbf_batch *batch, const float *rf_all, int reps, void *workspace,
                          size_t workspace_bytes, void *stream, int *best_tile, int *best_rchunk,
                          float *times_ms);

@@ -1,6 +1,6 @@
 """FP64 NumPy restatements used as checkers (test infrastructure only).
 
-* ``make_S`` restates ``__make_S_by_k__`` (pycbf/gpu/__core_bf_classes__.py:392-421):
+* ``make_S`` restates ``__make_S_by_k__`` (pycbf/gpu/__core_bf_classes__.py:366-394):
   the k x k operator mapping k samples to the not-a-knot cubic spline's first
   derivatives at those samples (unit spacing).
 * ``dmas`` restates the notebook DMAS formulas

@@ -462,10 +462,7 @@ int pycbf_batch_autotune(pycbf_batch *b, const float *rf_all, int reps, void *wo
 {
     if (!b || !rf_all || reps < 1) return fail(PYCBF_EINVAL, "bad autotune arguments");
     hipStream_t s = (hipStream_t)stream;
-    // receive chunks of 16 were the slowest at every tile on the 64-request
-    // batch of 128..512-square grids and 64..256 receive elements (20.9 /
-    // 18.4 / 17.8 ms for 16 / 32 / 64 at tile 8); they stay configurable
-    const int tiles[3] = {8, 16, 32}, chunks[3] = {32, 64, 128};
+    const int tiles[3] = {8, 16, 32}, chunks[4] = {16, 32, 64, 128};
     float best = 1e30f;
     int bt = 16, bc = 32;
     hipEvent_t e0, e1;
@@ -474,7 +471,7 @@ int pycbf_batch_autotune(pycbf_batch *b, const float *rf_all, int reps, void *wo
     int rc = pycbf_batch_configure(b, 16, 32, workspace, workspace_bytes, stream);
     if (!rc) rc = run_batch(b->plan, rf_all, workspace, workspace_bytes, s, false);  // tables
     for (int ti = 0; ti < 3 && !rc; ++ti)
-        for (int ci = 0; ci < 3 && !rc; ++ci) {
+        for (int ci = 0; ci < 4 && !rc; ++ci) {
             rc = pycbf_batch_configure(b, tiles[ti], chunks[ci], workspace, workspace_bytes, stream);
             if (rc) break;
             rc = run_batch(b->plan, rf_all, workspace, workspace_bytes, s, true);  // warm
@@ -487,7 +484,7 @@ int pycbf_batch_autotune(pycbf_batch *b, const float *rf_all, int reps, void *wo
             float ms = 0.0f;
             hipEventElapsedTime(&ms, e0, e1);
             ms /= (float)reps;
-            if (times_ms) times_ms[ti * 3 + ci] = ms;
+            if (times_ms) times_ms[ti * 4 + ci] = ms;
             if (ms < best) {
                 best = ms;
                 bt = tiles[ti];

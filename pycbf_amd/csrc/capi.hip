@@ -91,7 +91,9 @@ static DasPlan plan_das(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         break;
     case PYCBF_INTERP_CUBIC:
         p.gk = GK_CUBIC;
-        p.coef = align_up((size_t)p.nch * (nt + 1) * sizeof(float2), 256);
+        // 256 bytes of slack: the DMA-staged gather moves 16-byte chunks and may
+        // read one entry past a row's end
+        p.coef = align_up((size_t)p.nch * (nt + 1) * sizeof(float2) + 256, 256);
         p.uscr = spline_scratch_bytes(p.nch, (int32_t)nt);
         break;
     case PYCBF_INTERP_KORDER_CUBIC:

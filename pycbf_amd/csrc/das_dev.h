@@ -36,9 +36,10 @@ struct __attribute__((aligned(8))) Pair2 {
 };
 
 // Workgroups go to the 8 XCDs round robin by linear id.  With remap on, the
-// linear id is permuted so each XCD receives one contiguous run of pixel
-// tiles: tiles that are neighbours in the (spatially compact) processing order
-// read overlapping sample windows of every channel, and now share an L2.
+// linear id is permuted so each XCD receives one contiguous run of the
+// tile-fastest (tile, frame chunk, outer element) order: a slab of outer
+// elements over every tile, so the channels of an outer element are read
+// into one or two XCDs' L2 instead of all eight.
 __device__ __forceinline__ void block_coords(int remap, uint32_t &bx, uint32_t &by, uint32_t &bz)
 {
     const uint32_t gx = gridDim.x, gy = gridDim.y;

@@ -19,6 +19,9 @@
 // Delays and weights are read from tables (geo 0) or evaluated in registers
 // from the synthetic-point geometry (geo 2 / 3), both with the float32
 // operations of the reference (pycbf/gpu/__engines__.cu:382-470).
+//
+// The natural cubic with sums over 4 or 16 frames per pass runs
+// k_das_cubic_dma, which stages the same windows with LDS DMA (below).
 #include <math.h>
 #include <stdlib.h>
 
@@ -297,15 +300,18 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
                                 kb *= w;
                                 kc *= w;
                                 kd *= w;
+                                // every frame slot, present or not: the slots of
+                                // frames past nf lie in the buffer and feed
+                                // sums that are never stored, and straight-line
+                                // code lets the LDS reads of all frames overlap
     #pragma unroll
-                                for (int f = 0; f < FC; ++f)
-                                    if (f < nf) {
-                                        const float *Ef = E + f * WIN * 2;
-                                        float r = __builtin_fmaf(ka, Ef[0], acc[f]);
-                                        r = __builtin_fmaf(kb, Ef[2], r);
-                                        r = __builtin_fmaf(kc, Ef[1], r);
-                                        acc[f] = __builtin_fmaf(kd, Ef[3], r);
-                                    }
+                                for (int f = 0; f < FC; ++f) {
+                                    const float *Ef = E + f * WIN * 2;
+                                    float r = __builtin_fmaf(ka, Ef[0], acc[f]);
+                                    r = __builtin_fmaf(kb, Ef[2], r);
+                                    r = __builtin_fmaf(kc, Ef[1], r);
+                                    acc[f] = __builtin_fmaf(kd, Ef[3], r);
+                                }
                                 ok = false;  // already accumulated
                             }
                         }
@@ -370,6 +376,217 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
     }
 }
 
+// ---------------------------------------------------------------------------
+// natural-cubic gather over FC = 4 or 16 frames with sums (tx_and_rx, tx_only,
+// rx_only), staged by LDS DMA.  The windows come to LDS through gfx950's
+// global_load_lds_dwordx4: every lane moves one 16-byte chunk (two table
+// entries) straight into LDS at base + 16 * lane, without VGPRs and without
+// LDS writes by the waves.  A frame row of the window starts at the 16-byte
+// aligned entry at or just below its first entry (rows are 8-byte aligned; the
+// parity of row * (nt + 1) + lo says which), and each frame owns CH chunk
+// slots, so chunk q of the staged block lands at byte 16 q.  Per outer
+// element one pass computes every inner channel's window into an LDS table;
+// channel b + 1 is in flight while b is evaluated (two LDS buffers, one
+// barrier per staged channel).  A window wider than the slot takes direct
+// global gathers, as in k_das_lds.
+// ---------------------------------------------------------------------------
+constexpr int DMA_MAXB = 256;    // inner channels per outer element (window table)
+constexpr int DMA_CHUNKS = 768;  // 16-byte chunks per buffer (12 KB), all frames
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glb_void;
+
+template <int SUM, int FC, int GEO>
+__global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
+{
+    constexpr int CH = DMA_CHUNKS / FC;     // chunk slots per frame row: 48 (FC 16) or 192
+    constexpr int WINMAX = 2 * CH - 1;      // entries a frame row can always hold
+    constexpr int NI = DMA_CHUNKS / LDS_BLOCK;  // DMA instructions per wave and channel (3)
+    __shared__ __align__(16) float lds[2 * DMA_CHUNKS * 4];
+    __shared__ int2 wins[DMA_MAXB];
+
+    uint32_t bx, by, bz;
+    block_coords(A.xcd_remap, bx, by, bz);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t i = (int64_t)bx * LDS_BLOCK + tid;
+    const bool live = i < A.np;
+    const int64_t p = live ? (A.perm ? (int64_t)A.perm[i] : i) : 0;
+    const int f0 = by * FC;
+    const int nf = min(FC, A.nfr - f0);
+    const int64_t np = A.np;
+
+    const bool OUTER_TX = (SUM == PYCBF_SUM_RX_ONLY) || (SUM != PYCBF_SUM_TX_ONLY && A.outer_tx);
+    const int na = OUTER_TX ? A.ntx : A.nrx;
+    const int nb = OUTER_TX ? A.nrx : A.ntx;
+    const int a0 = bz * A.a_split;
+    const int a1 = min(na, a0 + A.a_split);
+    const int64_t nchan = (int64_t)A.ntx * A.nrx;
+    const int64_t rowlen = A.ch_stride / 2;  // table entries per channel row (nt + 1)
+    const int64_t fstride = nchan * A.ch_stride;
+    const float *bnd = A.bounds + (int64_t)bx * (A.ntx + A.nrx) * 2;
+    const int boff_a = OUTER_TX ? 0 : A.ntx;
+    const int boff_b = OUTER_TX ? A.ntx : 0;
+
+    float pv[3] = {0.0f, 0.0f, 0.0f};
+    if (GEO > 0 && live) {
+#pragma unroll
+        for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+    }
+    // DMA role of this lane: chunk q = (k * 4 + wave) * 64 + lane of the block
+    int dma_f[NI], dma_c[NI];
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const int q = (k * (LDS_BLOCK / 64) + wave) * 64 + lane;
+        dma_f[k] = q / CH;
+        dma_c[k] = q % CH;
+    }
+
+    float acc[FC];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) acc[f] = 0.0f;
+
+    for (int a = a0; a < a1; ++a) {
+        float ta = 0.0f, wa = 0.0f;
+        if (live) elem_eval<GEO>(A, OUTER_TX, a, p, pv, ta, wa);
+        const float amin = bnd[2 * (boff_a + a)], amax = bnd[2 * (boff_a + a) + 1];
+        for (int b = tid; b < nb; b += LDS_BLOCK) {
+            int lo, cnt;
+            channel_window<GK_CUBIC>(A, amin + bnd[2 * (boff_b + b)],
+                                     amax + bnd[2 * (boff_b + b) + 1], lo, cnt);
+            wins[b] = make_int2(lo, cnt);
+        }
+        __syncthreads();
+
+        // table row of frame f (of this pass) for inner channel b
+        auto row_of = [&](int b, int f) {
+            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
+            return ((int64_t)(f0 + f) * nchan + (int64_t)tx * A.nrx + rx);
+        };
+        auto staged = [&](int b) {
+            if (b >= nb) return false;
+            const int2 w = wins[b];
+            return w.y > 0 && w.y <= WINMAX;
+        };
+        // issue the DMA of channel b into buffer buf (workgroup-uniform condition)
+        auto dma = [&](int b, int buf) {
+            if (!staged(b)) return;
+            const int2 w = wins[b];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const int f = min(dma_f[k], nf - 1);  // chunks of absent frames reload frame nf-1
+                const int64_t r = row_of(b, f);
+                const int par = (int)((r * rowlen + w.x) & 1);
+                const int s0 = w.x - par;  // 16-byte aligned first entry (row r-1's last if -1)
+                const int nch = (w.x + w.y - s0 + 1) >> 1;
+                const int c = min(dma_c[k], nch - 1);
+                const float *src = A.src + (r * rowlen + s0 + 2 * c) * 2;
+                float *dst = lds + (buf * DMA_CHUNKS + (k * (LDS_BLOCK / 64) + wave) * 64) * 4;
+                __builtin_amdgcn_global_load_lds((glb_void *)src, (lds_void *)dst, 16, 0, 0);
+            }
+        };
+        auto compute = [&](int b, int buf) {
+            const int2 w = wins[b];
+            if (w.y == 0) return;
+            float tb = 0.0f, wb = 0.0f;
+            if (live) elem_eval<GEO>(A, !OUTER_TX, b, p, pv, tb, wb);
+            const float wgt = OUTER_TX ? wa * wb : wb * wa;  // apodtx * apodrx
+            const float x = OUTER_TX ? ta + tb : tb + ta;    // tautx + taurx
+            if (!(live && fabsf(wgt) >= A.thresh)) return;
+            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
+            if (w.y > WINMAX) {
+                float v[FC];
+                bool ok = false;
+                gather<GK_CUBIC, FC>(A, (int64_t)f0 * nchan + (int64_t)tx * A.nrx + rx, fstride,
+                                     nf, x, ok, v);
+                if (ok) {
+#pragma unroll
+                    for (int f = 0; f < FC; ++f)
+                        if (f < nf) acc[f] = __builtin_fmaf(wgt, v[f], acc[f]);
+                }
+                return;
+            }
+            int j = 0;
+            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j)) return;
+            const float xlead = A.t0 + A.dt * (float)(j + 1);
+            const float xlag = A.t0 + A.dt * (float)j;
+            float ka = (xlead - x) * A.rdt;
+            float kb = (x - xlag) * A.rdt;
+            float kc = __builtin_fmaf(ka * ka, ka, -ka);
+            float kd = __builtin_fmaf(kb * kb, kb, -kb);
+            ka *= wgt;
+            kb *= wgt;
+            kc *= wgt;
+            kd *= wgt;
+            // row parity of frame f: that of frame 0, flipped every frame when a
+            // frame's rows span an odd number of entries
+            const int par0 = (int)((row_of(b, 0) * rowlen + w.x) & 1);
+            const int fpar = (int)((nchan * rowlen) & 1);
+            // every frame slot: those of frames past nf hold copies of frame
+            // nf - 1 and feed sums that are never stored; straight-line code
+            // lets the LDS reads of all frames overlap
+            const float *L = lds + buf * DMA_CHUNKS * 4 + (j - w.x) * 2;
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+                const int par = par0 ^ (fpar & f);
+                const float *Ef = L + f * CH * 4 + par * 2;
+                float r = __builtin_fmaf(ka, Ef[0], acc[f]);
+                r = __builtin_fmaf(kb, Ef[2], r);
+                r = __builtin_fmaf(kc, Ef[1], r);
+                acc[f] = __builtin_fmaf(kd, Ef[3], r);
+            }
+        };
+
+        dma(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        if (staged(0)) __syncthreads();
+        for (int b = 0; b < nb; ++b) {
+            const int buf = b & 1;
+            dma(b + 1, buf ^ 1);
+            compute(b, buf);
+            // this wave's DMA of channel b + 1 has landed; the barrier makes
+            // every wave's land, and frees buffer buf for channel b + 2
+            __builtin_amdgcn_s_waitcnt(0);
+            if (staged(b) || staged(b + 1)) __syncthreads();
+        }
+        if (SUM == PYCBF_SUM_TX_ONLY || SUM == PYCBF_SUM_RX_ONLY) {
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+                if (live && f < nf) A.out[((int64_t)(f0 + f) * na + a) * np + p] = acc[f];
+                acc[f] = 0.0f;
+            }
+        }
+        __syncthreads();  // the window table is rewritten for the next outer element
+    }
+    if (SUM == PYCBF_SUM_TX_AND_RX && live) {
+        float *o = A.out + (int64_t)bz * A.nfr * np;
+#pragma unroll
+        for (int f = 0; f < FC; ++f)
+            if (f < nf) o[(int64_t)(f0 + f) * np + p] = acc[f];
+    }
+}
+
+template <int SUM, int FC>
+static void cubic_dma_geo(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    if (geo == 3)
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 3>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else if (geo == 2)
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 2>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 0>), grid, dim3(LDS_BLOCK), 0, s, a);
+}
+
+template <int FC>
+static void cubic_dma_sum(const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    switch (a.sumflag) {
+    case PYCBF_SUM_TX_ONLY: cubic_dma_geo<PYCBF_SUM_TX_ONLY, FC>(a.geo, a, grid, s); break;
+    case PYCBF_SUM_RX_ONLY: cubic_dma_geo<PYCBF_SUM_RX_ONLY, FC>(a.geo, a, grid, s); break;
+    default: cubic_dma_geo<PYCBF_SUM_TX_AND_RX, FC>(a.geo, a, grid, s); break;
+    }
+}
+
 template <int GK, int SUM, int FC>
 static void lds_geo(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
 {
@@ -403,6 +620,13 @@ static void lds_sum(int fc, const DasArgs &a, dim3 grid, hipStream_t s)
     }
 }
 
+// the natural-cubic DMA-staged gather (PYCBF_DAS_DMA=0 selects k_das_lds)
+static bool das_cubic_dma_enabled()
+{
+    const char *e = getenv("PYCBF_DAS_DMA");
+    return !(e && atoi(e) == 0);
+}
+
 // frames per workgroup for the staged gather: 16, 4 or 1
 int das_lds_frames_per_pass(int64_t nfr)
 {
@@ -417,7 +641,11 @@ int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStr
 {
     DasArgs a = a0;
     const int fc = das_lds_frames_per_pass(a.nfr);
-    a.xcd_remap = 0;
+    // XCD slabs: each XCD takes a contiguous run of (outer element, tile)
+    // work items, so a channel's table rows are pulled into one or two XCDs'
+    // L2 instead of all eight (16-frame headline: 13.3 -> 12.5 ms staged,
+    // 8.8 -> 8.4 ms DMA-staged); PYCBF_XCD_REMAP=0 restores round-robin
+    a.xcd_remap = 1;
     if (const char *e = getenv("PYCBF_XCD_REMAP")) a.xcd_remap = atoi(e);
     a.outer_tx = das_outer_is_tx(a.sumflag, a.ntx, a.nrx) ? 1 : 0;
     const int na = a.outer_tx ? a.ntx : a.nrx;
@@ -429,7 +657,18 @@ int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStr
     float *final_out = a.out;
     if (reduce) a.out = partial;
     dim3 grid((unsigned)das_tiles(a.np), (unsigned)((a.nfr + fc - 1) / fc), (unsigned)nsplit);
-    {
+    const int nb = a.outer_tx ? a.nrx : a.ntx;
+    const bool dma = gk == GK_CUBIC && a.sumflag != PYCBF_SUM_NONE && (fc == 4 || fc == 16) &&
+                     (a.sumflag == PYCBF_SUM_TX_AND_RX ? nb : (a.sumflag == PYCBF_SUM_TX_ONLY
+                                                                   ? a.ntx : a.nrx)) <= DMA_MAXB &&
+                     das_cubic_dma_enabled();
+    if (dma) {
+        ProfScope prof("das_gather", s);
+        if (fc == 16)
+            cubic_dma_sum<16>(a, grid, s);
+        else
+            cubic_dma_sum<4>(a, grid, s);
+    } else {
         ProfScope prof("das_gather", s);
         switch (gk) {
         case GK_NEAREST: lds_sum<GK_NEAREST>(fc, a, grid, s); break;

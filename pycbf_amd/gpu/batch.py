@@ -168,15 +168,15 @@ class RaggedBatch:
         """Times every (tile, receive chunk) pair on this batch and keeps the
         fastest; returns (tile, rchunk, {(tile, rchunk): ms})."""
         bt, bc = ct.c_int(), ct.c_int()
-        times = (ct.c_float * 9)()
+        times = (ct.c_float * 12)()
         rc = E.lib.pycbf_batch_autotune(self._h, E.ptr(self.rf_all), int(reps), E.ptr(self._ws),
                                         self._ws.numel(), E.stream_of(self.device), ct.byref(bt),
                                         ct.byref(bc), ct.cast(times, ct.c_void_p))
         E.check(rc, "pycbf_batch_autotune")
         self.tile, self.rchunk = bt.value, bc.value
         self._graph = None
-        grid = {(t, c): times[i * 3 + j] for i, t in enumerate((8, 16, 32))
-                for j, c in enumerate((32, 64, 128))}
+        grid = {(t, c): times[i * 4 + j] for i, t in enumerate((8, 16, 32))
+                for j, c in enumerate((16, 32, 64, 128))}
         return self.tile, self.rchunk, grid
 
     def _launch(self):

@@ -37,7 +37,7 @@ def _torch():
 def make_S_by_k(k: int) -> np.ndarray:
     """k x k operator mapping k unit-spaced samples to the derivatives of their
     not-a-knot cubic spline (Praesius & Jensen, UFFC-JS 2024); same matrices
-    as pycbf/gpu/__core_bf_classes__.py:392-421."""
+    as pycbf/gpu/__core_bf_classes__.py:366-394."""
     main = np.full(k, 4.0)
     main[[0, -1]] = 1.0
     upper = np.ones(k - 1)

@@ -211,3 +211,33 @@ def test_headline_plain_comparator(geo, rf20):
     for f in range(2):
         ref = _oracle_frame(geo, rf20[f].cpu().numpy())
         assert_close_to_peak(got[f], ref, 1e-5, f"plain comparator frame {f}")
+
+
+def test_headline_staged_register_gather(geo, rf20):
+    """The register-staged gather (PYCBF_DAS_DMA=0, k_das_lds) on the cubic
+    headline, 16 frames, and with round-robin workgroup placement
+    (PYCBF_XCD_REMAP=0) for 2 frames: both against the oracle."""
+    import torch
+    print("headline cubic, register-staged gather (PYCBF_DAS_DMA=0), 16 frames", flush=True)
+    sub = torch.as_tensor(geo["sub"], device="cuda")
+    os.environ["PYCBF_DAS_DMA"] = "0"
+    try:
+        out = _bf(geo, {"kind": "cubic"})(rf20[:16], out_as_numpy=False)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PYCBF_DAS_DMA"]
+    got = out[:, sub].cpu().numpy()
+    for f in (0, 7, 15):
+        ref = _oracle_frame(geo, rf20[f].cpu().numpy())
+        assert_close_to_peak(got[f], ref, 1e-5, f"register-staged frame {f}")
+    os.environ["PYCBF_XCD_REMAP"] = "0"
+    try:
+        out = _bf(geo, {"kind": "cubic"})(rf20[:2], out_as_numpy=False)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["PYCBF_XCD_REMAP"]
+    got = out[:, sub].cpu().numpy()
+    for f in range(2):
+        ref = _oracle_frame(geo, rf20[f].cpu().numpy())
+        assert_close_to_peak(got[f], ref, 1e-5, f"round-robin placement frame {f}")
+
