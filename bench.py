@@ -610,6 +610,9 @@ def main():
     ap.add_argument("--adjoint-reps", type=int, default=10)
     ap.add_argument("--data", choices=["sim", "noise"], default="sim",
                     help="noise: seeded Gaussian RF of the same shape (profiling runs)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain-GPU comparators and the bandwidth measurements "
+                         "(profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB before timing "
                          "(reference kernel for scaling FETCH_SIZE / WRITE_SIZE)")
@@ -685,17 +688,19 @@ def main():
     # on-chip: the stencil reads (16 B per contributing sample and frame) the
     # gather serves from LDS, against the measured LDS read bandwidth
     contrib = count_contributing(bf, 1e-2)
-    peaks = {k: round(E.measure_bandwidth(k, dev), 1) for k in ("hbm", "l2", "lds")}
-    log(f"measured read bandwidth GB/s: {peaks}; contributing samples per frame {contrib} "
-        f"({contrib / (NTX * NRX * npx):.3f} of all)")
     lds_bytes = 16.0 * contrib * args.frames
     lds_achieved = lds_bytes / (gather_ms * 1e-3) / 1e9
-
-    # plain-GPU comparators on the same workload and data
-    plain_ms, plain_err = time_variant("PYCBF_DAS_PLAIN", "1", params, rf, out, 2)
-    direct_ms, direct_err = time_variant("PYCBF_DAS_LDS", "0", params, rf, out, 2)
-    log(f"thread-per-sample comparator {plain_ms:.2f} ms/step (err {plain_err:.2e}); direct "
-        f"gather {direct_ms:.2f} ms/step (err {direct_err:.2e})")
+    peaks = {"hbm": None, "l2": None, "lds": None}
+    plain_ms = plain_err = direct_ms = direct_err = None
+    if not args.no_compare:
+        peaks = {k: round(E.measure_bandwidth(k, dev), 1) for k in ("hbm", "l2", "lds")}
+        log(f"measured read bandwidth GB/s: {peaks}; contributing samples per frame {contrib} "
+            f"({contrib / (NTX * NRX * npx):.3f} of all)")
+        # plain-GPU comparators on the same workload and data
+        plain_ms, plain_err = time_variant("PYCBF_DAS_PLAIN", "1", params, rf, out, 2)
+        direct_ms, direct_err = time_variant("PYCBF_DAS_LDS", "0", params, rf, out, 2)
+        log(f"thread-per-sample comparator {plain_ms:.2f} ms/step (err {plain_err:.2e}); direct "
+            f"gather {direct_ms:.2f} ms/step (err {direct_err:.2e})")
 
     cpu = parity = None
     if not args.no_cpu:
@@ -755,22 +760,24 @@ def main():
                      "bytes": "spline table read once + pixel positions and order + images"},
         "roofline_onchip": {"bound": "lds", "achieved": round(lds_achieved, 1),
                             "peak": peaks["lds"], "unit": "GB/s",
-                            "frac": round(lds_achieved / peaks["lds"], 4),
+                            "frac": round(lds_achieved / peaks["lds"], 4) if peaks["lds"] else None,
                             "bytes": "16 B per contributing sample and frame",
                             "contributing_samples_per_frame": contrib,
                             "all_samples_per_frame": NTX * NRX * npx},
         "peaks_measured_gbps": peaks,
-        "plain_gpu": {"value": round(samples / (plain_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
-                      "ms_per_step": round(plain_ms, 3),
-                      "max_err_rel_peak_vs_headline": float(f"{plain_err:.3g}"),
-                      "kernel": "thread per (frame, tx, rx, px) sample with one float atomic "
-                                "each: pycbf's GPU launch scheme (PYCBF_DAS_PLAIN=1)"},
-        "direct_gpu": {"value": round(samples / (direct_ms * 1e-3) / 1e9, 2),
-                       "unit": "Gsamples/s", "ms_per_step": round(direct_ms, 3),
-                       "max_err_rel_peak_vs_headline": float(f"{direct_err:.3g}"),
-                       "kernel": "lane per pixel over every channel, delay tables, direct "
-                                 "global gathers (PYCBF_DAS_LDS=0)"},
-        "speedup_vs_plain_gpu": round(plain_ms / ms_step, 2),
+        "plain_gpu": None if plain_ms is None else {
+            "value": round(samples / (plain_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
+            "ms_per_step": round(plain_ms, 3),
+            "max_err_rel_peak_vs_headline": float(f"{plain_err:.3g}"),
+            "kernel": "thread per (frame, tx, rx, px) sample with one float atomic each: "
+                      "pycbf's GPU launch scheme (PYCBF_DAS_PLAIN=1)"},
+        "direct_gpu": None if direct_ms is None else {
+            "value": round(samples / (direct_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
+            "ms_per_step": round(direct_ms, 3),
+            "max_err_rel_peak_vs_headline": float(f"{direct_err:.3g}"),
+            "kernel": "lane per pixel over every channel, delay tables, direct global "
+                      "gathers (PYCBF_DAS_LDS=0)"},
+        "speedup_vs_plain_gpu": None if plain_ms is None else round(plain_ms / ms_step, 2),
         "cpu_baseline": cpu,
         "parity": parity,
         "phases_ms": {"das_gather": round(gather_ms, 4),
