@@ -508,7 +508,7 @@ def adjoint_leg(dev, reps):
     }
 
 
-def batch_leg(dev, runs, speckle):
+def batch_leg(dev, runs, speckle, measure_peaks=True):
     """Ragged batch: 64 seeded requests, grid side in {128, 192, 256, 384, 512},
     Ntx in {1, 11, 31, 75}, Nrx in {64, 128, 192, 256}, Nt = 4096; cubic DAS
     for 48 requests and SLSC (lags 1..20) for 16; one persistent gather launch
@@ -555,18 +555,33 @@ def batch_leg(dev, runs, speckle):
     lat = batch.latencies_ms()
     log(f"batch: {ms:.2f} ms/run, {batch.samples / (ms * 1e-3) / 1e9:.1f} Gsamples/s, latency p50 "
         f"{np.percentile(lat, 50):.2f} p99 {np.percentile(lat, 99):.2f} ms")
-    # algorithmic bytes of the gather, per request: 16 B per cubic sample, the
-    # RF once, the pixel positions, and the output (summed image, or the
-    # per-receive images of an SLSC request)
-    algo = 0.0
-    for (steers, xele, ntx, nrx, side), kind in zip(specs, kinds):
+    # HBM bound of the gather on its compulsory bytes, per request: the
+    # (y, h*y2) table once (8 B per entry, nt + 1 entries per channel), the
+    # pixel positions, and the output written once (summed image, or the
+    # per-receive images of an SLSC request).  The 16 B stencil of every
+    # contributing sample comes from L1/L2 (the batch kernel gathers directly,
+    # no LDS staging): the on-chip bound is that traffic against the measured
+    # L2 read bandwidth.
+    from pycbf_amd.gpu import SyntheticBeamformer
+    hbm = 0.0
+    contrib = 0
+    for (steers, xele, ntx, nrx, side), req, kind in zip(specs, reqs, kinds):
         npx = side * side
-        algo += (16.0 * ntx * nrx * npx + 4.0 * ntx * nrx * NT + 8.0 * npx
-                 + 4.0 * npx * (nrx if kind == "slsc" else 1))
+        hbm += (8.0 * ntx * nrx * (NT + 1) + 8.0 * npx
+                + 4.0 * npx * (nrx if kind == "slsc" else 1))
+        bfr = SyntheticBeamformer(**req.params, t0=0.0, dt=1.0 / 40e6, nt=NT,
+                                  interp={"kind": "cubic"}, sumtype="tx_and_rx", device=dev)
+        contrib += count_contributing(bfr, req.thresh)
+        del bfr
     g_ms = grid[(tile, rchunk)]
-    g_achieved = algo / (g_ms * 1e-3) / 1e9
-    log(f"batch gather roofline: {algo / 1e9:.1f} GB algorithmic, {g_achieved:.0f} GB/s "
-        f"({g_achieved / HBM_PEAK_GBPS:.3f} of HBM)")
+    g_achieved = hbm / (g_ms * 1e-3) / 1e9
+    l2_bytes = 16.0 * contrib
+    l2_achieved = l2_bytes / (g_ms * 1e-3) / 1e9
+    l2_peak = E.measure_bandwidth("l2", dev) if measure_peaks else None
+    log(f"batch gather: {hbm / 1e9:.2f} GB compulsory, {g_achieved:.0f} GB/s "
+        f"({g_achieved / HBM_PEAK_GBPS:.3f} of HBM); stencil {l2_bytes / 1e9:.1f} GB from cache "
+        f"({contrib / batch.samples:.3f} of samples contribute), {l2_achieved:.0f} GB/s"
+        + (f" ({l2_achieved / l2_peak:.3f} of measured L2)" if l2_peak else ""))
     res = {
         "workload": "ragged batch of 64 seeded requests (48 cubic DAS, 16 SLSC lags 1..20), grids "
                     "128..512 square, Ntx 1..75, Nrx 64..256, Nt 4096; one persistent gather "
@@ -579,7 +594,13 @@ def batch_leg(dev, runs, speckle):
                        "max": round(float(lat.max()), 3)},
         "roofline": {"bound": "hbm", "achieved": round(g_achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(g_achieved / HBM_PEAK_GBPS, 4),
-                     "kernel": "batch_gather", "kernel_ms": round(g_ms, 3)},
+                     "kernel": "batch_gather", "kernel_ms": round(g_ms, 3),
+                     "bytes": "spline tables read once + pixel positions + outputs"},
+        "roofline_onchip": {"bound": "l2", "achieved": round(l2_achieved, 1),
+                            "peak": round(l2_peak, 1) if l2_peak else None, "unit": "GB/s",
+                            "frac": round(l2_achieved / l2_peak, 4) if l2_peak else None,
+                            "bytes": "16 B per contributing sample",
+                            "contributing_samples": int(contrib)},
         "autotune": {"tile": tile, "rchunk": rchunk,
                      "gather_ms": {f"{t}x{t}/{c}": round(v, 3) for (t, c), v in grid.items()}},
         "data": f"seeded channel data, 50 points + {speckle} speckle scatterers per request",
@@ -734,7 +755,8 @@ def main():
         slsc_res, demod_res = slsc_leg(dev, args.slsc_reps, not args.no_cpu)
     adj_res = None if args.no_adjoint else adjoint_leg(dev, args.adjoint_reps)
     sos_res = None if args.no_sos else sos_leg(dev, args.sos_steps)
-    batch_res = None if args.no_batch else batch_leg(dev, args.batch_runs, args.batch_speckle)
+    batch_res = None if args.no_batch else batch_leg(dev, args.batch_runs, args.batch_speckle,
+                                                     not args.no_compare)
 
     line = {
         "metric": METRIC,

@@ -253,7 +253,8 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 int pycbf_dmas(const float *im, int64_t nfr, int64_t nsep, int64_t np, const int32_t *lags,
                int nlag, int dmastype, int sumlags, float *out, void *stream);
 
-/* Bytes of workspace pycbf_das_tabbed_bwd needs (cubic: spline table and
+/* Bytes of workspace pycbf_das_tabbed_bwd and pycbf_das_synthetic_bwd need
+ * (cubic: spline table and
  * cotangent bins of every channel; korder_cubic: Hermite table and the
  * cotangents of its four coefficients, as for akima and makima; nearest with
  * param = usf > 1: the upsampled trace and its cotangents; all kinds:
@@ -289,6 +290,29 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                          int sumflag, const float *gout, float *grf, float *gtautx,
                          float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
                          size_t workspace_bytes, void *stream);
+
+/* Adjoint of pycbf_das_synthetic (no reference counterpart; the forward it
+ * differentiates is pycbf/gpu/__engines__.cu:382-470 (geometry) with
+ * :620-694 (synthetic DAS)).  Same gradients as pycbf_das_tabbed_bwd, but no
+ * delay or apodisation table exists: every kernel re-evaluates the delays and
+ * weights of its (element, pixel) pairs in registers from the packs
+ * (pycbf_synthetic_pack) and the pixel positions pvec (np, ndim), with the
+ * forward's float32 operations, so grf, gtautx, gapodtx, gtaurx and gapodrx
+ * equal what pycbf_das_tabbed_bwd gives on pycbf_synthetic_tables' tables.
+ * gc0 (device, one float; NULL: not computed) receives d loss / d c0 for a
+ * geometry whose every delay is a path length over c0, the transmit time
+ * intercept t0tx included (pycbf's helpers compute it as a distance over the
+ * sound speed, pycbf/helpers/__full_aperture_2D__.py:259,300,346): the sum
+ * over (element, pixel) of the delay gradient times d tau / d c0 = -tau / c0,
+ * accumulated in double in a fixed order (deterministic).  The weights depend
+ * on angles only (their gradient is zero).  Pixels are taken in their natural
+ * order (no processing order).  Workspace: pycbf_das_bwd_workspace_bytes. */
+int pycbf_das_synthetic_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                            const float *txpack, const float *rxpack, float c0, const float *pvec,
+                            int64_t np, int interp_kind, int param, const float *S, float thresh,
+                            int sumflag, const float *gout, float *grf, float *gtautx,
+                            float *gapodtx, float *gtaurx, float *gapodrx, float *gc0,
+                            void *workspace, size_t workspace_bytes, void *stream);
 
 /* Straight-ray delays through a slowness map (no reference counterpart: the
  * reference's examples/rough_sos_correction.ipynb uses an external eikonal

@@ -35,12 +35,37 @@ from __future__ import annotations
 import torch
 
 
+def _solve_141(rhs: torch.Tensor) -> torch.Tensor:
+    """x with tridiag(1, 4, 1) x = rhs along the last axis (no autograd): the
+    Thomas algorithm, vectorised over the leading axes (the matrix is strictly
+    diagonally dominant, so no pivoting is needed)."""
+    import numpy as np
+    m = rhs.shape[-1]
+    cp = np.empty(m)
+    cp[0] = 0.25
+    for i in range(1, m):
+        cp[i] = 1.0 / (4.0 - cp[i - 1])
+    d = torch.empty_like(rhs)
+    d[..., 0] = rhs[..., 0] * 0.25
+    for i in range(1, m):
+        d[..., i] = (rhs[..., i] - d[..., i - 1]) * float(cp[i])
+    for i in range(m - 2, -1, -1):
+        d[..., i] -= float(cp[i]) * d[..., i + 1]
+    return d
+
+
 def natural_y2(y: torch.Tensor, dt: float) -> torch.Tensor:
-    """Second derivatives of the natural cubic spline through y (..., n)."""
+    """Second derivatives of the natural cubic spline through y (..., n).
+    Dense solve (differentiable); without autograd and for long traces of
+    many channels, the tridiagonal sweep (_solve_141, same system)."""
     n = y.shape[-1]
     if n < 3:
         return torch.zeros_like(y)
     m = n - 2
+    if not (y.requires_grad and torch.is_grad_enabled()) and y.numel() > (1 << 22):
+        rhs = (y[..., 2:] - 2.0 * y[..., 1:-1] + y[..., :-2]) * (6.0 / (dt * dt))
+        z = torch.zeros(y.shape[:-1] + (1,), dtype=y.dtype, device=y.device)
+        return torch.cat([z, _solve_141(rhs), z], dim=-1)
     T = (torch.diag(torch.full((m,), 4.0, dtype=y.dtype, device=y.device))
          + torch.diag(torch.ones(m - 1, dtype=y.dtype, device=y.device), 1)
          + torch.diag(torch.ones(m - 1, dtype=y.dtype, device=y.device), -1))

@@ -591,7 +591,9 @@ int pycbf_debug_indices(const pycbf_rfinfo *info, const float *tautx, const floa
 // ------------------------------------------------------------- backward ---
 struct BwdPlan {
     size_t bounds = 0, coef = 0, uscr = 0, gy = 0, ge = 0, gc2 = 0, gc3 = 0, mbits = 256;
-    size_t y2 = 0, up = 0, gu = 0, total = 0;  // nearest, usf > 1
+    size_t y2 = 0, up = 0, gu = 0;  // nearest, usf > 1
+    size_t gc0 = 0;                 // partial sums of d loss / d c0 (synthetic geometry)
+    size_t total = 0;
 };
 
 static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int kind, int param)
@@ -600,6 +602,7 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
     const int64_t nt = info->tinfo.nx;
     const int64_t nch = nfr * info->ntx * info->nrx;
     b.bounds = align_up((size_t)das_tiles(np) * (size_t)(info->ntx + info->nrx) * 2 * 4, 256);
+    b.gc0 = align_up((size_t)das_tiles(np) * (size_t)(info->ntx + info->nrx) * 8, 256);
     if (kind == PYCBF_INTERP_CUBIC) {
         b.coef = align_up((size_t)nch * (size_t)(nt + 1) * 8, 256);
         b.uscr = spline_scratch_bytes(nch, (int32_t)nt);
@@ -616,7 +619,7 @@ static BwdPlan plan_bwd(const pycbf_rfinfo *info, int64_t nfr, int64_t np, int k
         b.gy = b.ge = align_up((size_t)nch * (size_t)(nt + 1) * 4, 256);
     }
     b.total = b.bounds + b.coef + b.uscr + b.gy + b.ge + b.gc2 + b.gc3 + b.mbits + b.y2 + b.up +
-              b.gu + 11 * 256;
+              b.gu + b.gc0 + 12 * 256;
     return b;
 }
 
@@ -627,13 +630,11 @@ size_t pycbf_das_bwd_workspace_bytes(const pycbf_rfinfo *info, int64_t nfr, int6
     return plan_bwd(info, nfr, np, interp_kind, param).total;
 }
 
-int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
-                         const float *tautx, const float *apodtx, const float *taurx,
-                         const float *apodrx, const float *tautx_lo, const float *taurx_lo,
-                         int64_t np, int interp_kind, int param, const float *S, float thresh,
-                         int sumflag, const float *gout, float *grf, float *gtautx,
-                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
-                         size_t workspace_bytes, void *stream)
+// Argument checks shared by the two adjoints; returns the kernel parameter
+// (k, usf or 1) through kparam.
+static int bwd_validate(const pycbf_rfinfo *info, const float *rf, int64_t nfr, int64_t np,
+                        int interp_kind, int param, const float *S, int sumflag,
+                        const float *gout, int &kparam)
 {
     if (interp_kind != PYCBF_INTERP_NEAREST && interp_kind != PYCBF_INTERP_LINEAR &&
         interp_kind != PYCBF_INTERP_CUBIC && interp_kind != PYCBF_INTERP_KORDER_CUBIC &&
@@ -642,32 +643,36 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
                     "the adjoint supports nearest, linear, cubic, korder_cubic, akima and makima "
                     "(got %d)",
                     interp_kind);
-    const int kparam =
+    kparam =
         (interp_kind == PYCBF_INTERP_KORDER_CUBIC || interp_kind == PYCBF_INTERP_NEAREST) ? param
                                                                                           : 1;
     int rc = validate_common(info, rf, nfr, np, interp_kind, kparam,
                              interp_kind == PYCBF_INTERP_KORDER_CUBIC ? S : nullptr, sumflag, gout);
     if (rc) return rc;
-    if (!tautx || !apodtx || !taurx || !apodrx)
-        return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
     if (nfr > 65535 || info->ntx > 65535) return fail(PYCBF_EINVAL, "nfr and ntx must be <= 65535");
-    const BwdPlan plan = plan_bwd(info, nfr, np, interp_kind, kparam);
+    return 0;
+}
+
+// The adjoint pipeline once the delays and weights are fixed: b carries the
+// tables (b.geo 0) or the synthetic geometry (b.geo 2 / 3); tb the same
+// source for the per-tile delay ranges.  Prefilters the channel data as the
+// forward does, runs the adjoint kernels and transposes the prefilter.
+static int bwd_run(const pycbf_rfinfo *info, const float *rf, int64_t nfr, int64_t np,
+                   int interp_kind, int param, const float *S, float thresh, int sumflag,
+                   const float *gout, BwdArgs b, DasArgs tb, float *grf, float *gtautx,
+                   float *gapodtx, float *gtaurx, float *gapodrx, float *gc0, void *workspace,
+                   size_t workspace_bytes, hipStream_t s)
+{
+    const BwdPlan plan = plan_bwd(info, nfr, np, interp_kind, param);
     if (!workspace || workspace_bytes < plan.total)
         return fail(PYCBF_EINVAL, "workspace of %zu bytes needed, %zu given", plan.total,
                     workspace_bytes);
-    hipStream_t s = (hipStream_t)stream;
+    int rc = 0;
     Carver cv(workspace, workspace_bytes);
     const int nt = info->tinfo.nx;
     const float t0 = info->tinfo.x0, dt = info->tinfo.dx;
     const int64_t nch = nfr * info->ntx * info->nrx;
 
-    BwdArgs b{};
-    b.ttx = tautx;
-    b.atx = apodtx;
-    b.trx = taurx;
-    b.arx = apodrx;
-    b.ttx_lo = tautx_lo;
-    b.trx_lo = taurx_lo;
     b.np = np;
     b.ntx = (int)info->ntx;
     b.nrx = (int)info->nrx;
@@ -684,6 +689,7 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
     float *bounds = (float *)cv.take(plan.bounds);
     b.bounds = bounds;
     b.mbits = (unsigned *)cv.take(plan.mbits);
+    b.gc0part = (double *)cv.take(plan.gc0);
     if (interp_kind == PYCBF_INTERP_CUBIC) {
         float2 *coef = (float2 *)cv.take(plan.coef);
         float *u = (float *)cv.take(plan.uscr);
@@ -734,30 +740,88 @@ int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
         b.gstride = nt;
     }
     if (grf) {
-        DasArgs a{};
-        a.ttx = tautx;
-        a.atx = apodtx;
-        a.trx = taurx;
-        a.arx = apodrx;
-        a.np = np;
-        a.ntx = b.ntx;
-        a.nrx = b.nrx;
-        a.thresh = thresh;
-        a.geo = 0;
-        if ((rc = launch_tile_bounds(a, bounds, s))) return rc;
+        tb.np = np;
+        tb.ntx = b.ntx;
+        tb.nrx = b.nrx;
+        tb.thresh = thresh;
+        tb.perm = nullptr;
+        if ((rc = launch_tile_bounds(tb, bounds, s))) return rc;
     }
     const int gk = interp_kind == PYCBF_INTERP_NEAREST  ? GK_NEAREST
                    : interp_kind == PYCBF_INTERP_LINEAR ? GK_LINEAR
                    : interp_kind == PYCBF_INTERP_CUBIC  ? GK_CUBIC
                    : interp_kind == PYCBF_INTERP_KORDER_CUBIC ? GK_KORDER
                                                               : GK_AKIMA;
-    if ((rc = launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, s))) return rc;
+    if ((rc = launch_das_bwd(gk, b, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s))) return rc;
     if (grf && interp_kind == PYCBF_INTERP_NEAREST && param > 1) {
         float *gy = (float *)cv.take(plan.gy);
         float *ge = (float *)cv.take(plan.ge);
         return launch_upsample_adjoint(b.gy, nch, nt, param, t0, dt, gy, ge, grf, s);
     }
     return 0;
+}
+
+int pycbf_das_tabbed_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                         const float *tautx, const float *apodtx, const float *taurx,
+                         const float *apodrx, const float *tautx_lo, const float *taurx_lo,
+                         int64_t np, int interp_kind, int param, const float *S, float thresh,
+                         int sumflag, const float *gout, float *grf, float *gtautx,
+                         float *gapodtx, float *gtaurx, float *gapodrx, void *workspace,
+                         size_t workspace_bytes, void *stream)
+{
+    int kparam = 1;
+    int rc = bwd_validate(info, rf, nfr, np, interp_kind, param, S, sumflag, gout, kparam);
+    if (rc) return rc;
+    if (!tautx || !apodtx || !taurx || !apodrx)
+        return fail(PYCBF_EINVAL, "delay / apodisation tables must be device pointers");
+    BwdArgs b{};
+    b.ttx = tautx;
+    b.atx = apodtx;
+    b.trx = taurx;
+    b.arx = apodrx;
+    b.ttx_lo = tautx_lo;
+    b.trx_lo = taurx_lo;
+    b.geo = 0;
+    DasArgs tb{};
+    tb.ttx = tautx;
+    tb.atx = apodtx;
+    tb.trx = taurx;
+    tb.arx = apodrx;
+    tb.geo = 0;
+    return bwd_run(info, rf, nfr, np, interp_kind, kparam, S, thresh, sumflag, gout, b, tb, grf,
+                   gtautx, gapodtx, gtaurx, gapodrx, nullptr, workspace, workspace_bytes,
+                   (hipStream_t)stream);
+}
+
+int pycbf_das_synthetic_bwd(const pycbf_rfinfo *info, const float *rf, int64_t nfr,
+                            const float *txpack, const float *rxpack, float c0, const float *pvec,
+                            int64_t np, int interp_kind, int param, const float *S, float thresh,
+                            int sumflag, const float *gout, float *grf, float *gtautx,
+                            float *gapodtx, float *gtaurx, float *gapodrx, float *gc0,
+                            void *workspace, size_t workspace_bytes, void *stream)
+{
+    int kparam = 1;
+    int rc = bwd_validate(info, rf, nfr, np, interp_kind, param, S, sumflag, gout, kparam);
+    if (rc) return rc;
+    if (info->ndim != 2 && info->ndim != 3)
+        return fail(PYCBF_EINVAL, "synthetic geometry needs ndim 2 or 3");
+    if (!txpack || !rxpack || !pvec) return fail(PYCBF_EINVAL, "NULL geometry pointer");
+    if (!(c0 > 0.0f) || !isfinite(c0)) return fail(PYCBF_EINVAL, "c0 must be finite and > 0");
+    BwdArgs b{};
+    b.geo = info->ndim;
+    b.txpack = txpack;
+    b.rxpack = rxpack;
+    b.pvec = pvec;
+    b.c0 = c0;
+    DasArgs tb{};
+    tb.geo = info->ndim;
+    tb.txpack = txpack;
+    tb.rxpack = rxpack;
+    tb.pvec = pvec;
+    tb.c0 = c0;
+    return bwd_run(info, rf, nfr, np, interp_kind, kparam, S, thresh, sumflag, gout, b, tb, grf,
+                   gtautx, gapodtx, gtaurx, gapodrx, gc0, workspace, workspace_bytes,
+                   (hipStream_t)stream);
 }
 
 // ------------------------------------------------------- speed of sound ---

@@ -153,9 +153,17 @@ struct BwdArgs {
     const float *ttx_lo, *trx_lo;  // delay residuals (may be null): the delay gradient
                                    // evaluates the interpolant's derivative at
                                    // (tautx + tautx_lo) + (taurx + taurx_lo) in double
+    int geo;                       // 0: the tables above; 2 / 3: synthetic geometry of that
+                                   // ndim, evaluated in registers (tables unused)
+    const float *txpack, *rxpack, *pvec;  // synthetic geometry (geo > 0), pvec (np, geo)
+    float c0;
+    double *gc0part;  // per-workgroup partial sums of d loss / d c0 (geo > 0), or null
+    int gc0_off;      // first partial of this launch
 };
+// gc0 (geo > 0, may be null): d loss / d c0, one float, reduced in a fixed order
+// from a.gc0part ((ntx + nrx) * das_tiles(np) doubles)
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
-                   float *gapodrx, float *grf, hipStream_t s);
+                   float *gapodrx, float *grf, float *gc0, hipStream_t s);
 // nearest with usf > 1: cotangents gu of the upsampled trace (nch, usf (nt-1)
 // + 1) -> spline-table cotangents gy, ge (nch, nt + 1) -> grf (nch, nt)
 int launch_upsample_adjoint(const float *gu, int64_t nch, int32_t nt, int32_t usf, float t0,

@@ -141,31 +141,67 @@ __device__ __forceinline__ int64_t gout_index(int sumflag, int f, int tx, int rx
 
 }  // namespace
 
-// one lane per (outer element, pixel); outer axis tx (OUTER_TX) or rx
-template <int GK, int OUTER_TX>
+// pixel position p of the synthetic geometry (GEO = ndim; nothing for tables)
+template <int GEO>
+__device__ __forceinline__ void bwd_pixel(const BwdArgs &A, int64_t p, float (&pv)[3])
+{
+    pv[0] = pv[1] = pv[2] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+}
+
+// delay and weight of element e of the transmit (tx) or receive axis at pixel
+// p: the tables (GEO 0), or the synthetic geometry evaluated in registers with
+// the forward's float32 operations (GEO = ndim)
+template <int GEO>
+__device__ __forceinline__ void bwd_elem(const BwdArgs &A, bool tx, int e, int64_t p,
+                                         const float (&pv)[3], float &tau, float &apod)
+{
+    if (GEO == 0) {
+        const int64_t o = (int64_t)e * A.np + p;
+        tau = tx ? A.ttx[o] : A.trx[o];
+        apod = tx ? A.atx[o] : A.arx[o];
+    } else if (tx) {
+        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
+    } else {
+        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
+    }
+}
+
+// one lane per (outer element, pixel); outer axis tx (OUTER_TX) or rx.  With
+// the synthetic geometry (GEO > 0) and A.gc0part set, each workgroup also
+// writes its share of d loss / d c0 = sum over (element, pixel) of the delay
+// gradient times d tau / d c0 = -tau / c0: every delay of the geometry is a
+// path length over c0, the transmit's time intercept included (pycbf's
+// helpers compute it as a distance over the sound speed,
+// pycbf/helpers/__full_aperture_2D__.py:259,300,346).
+template <int GK, int OUTER_TX, int GEO>
 __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
 {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int o = blockIdx.y;
-    if (p >= A.np) return;
+    const bool act = p < A.np;
     const int ninner = OUTER_TX ? A.nrx : A.ntx;
-    const float *to = OUTER_TX ? A.ttx : A.trx;
-    const float *ao = OUTER_TX ? A.atx : A.arx;
-    const float *ti = OUTER_TX ? A.trx : A.ttx;
-    const float *ai = OUTER_TX ? A.arx : A.atx;
-    const float tau_o = to[(int64_t)o * A.np + p];
-    const float ap_o = ao[(int64_t)o * A.np + p];
+    float pv[3];
+    bwd_pixel<GEO>(A, act ? p : 0, pv);
+    float tau_o = 0.0f, ap_o = 0.0f;
+    if (act) bwd_elem<GEO>(A, OUTER_TX, o, p, pv, tau_o, ap_o);
     // delay residuals: the derivative is taken at the position in double
     const float *lo_o_t = OUTER_TX ? A.ttx_lo : A.trx_lo;
     const float *lo_i_t = OUTER_TX ? A.trx_lo : A.ttx_lo;
-    const bool fine = (A.ttx_lo || A.trx_lo) && GK != GK_NEAREST;
-    const double lo_o = lo_o_t ? (double)lo_o_t[(int64_t)o * A.np + p] : 0.0;
+    const bool fine = GEO == 0 && (A.ttx_lo || A.trx_lo) && GK != GK_NEAREST;
+    const double lo_o = (fine && lo_o_t && act) ? (double)lo_o_t[(int64_t)o * A.np + p] : 0.0;
     float gt = 0.0f, ga = 0.0f;
-    for (int i = 0; i < ninner; ++i) {
-        const float ap_i = ai[(int64_t)i * A.np + p];
+    for (int i = 0; act && i < ninner; ++i) {
+        float tau_i = 0.0f, ap_i = 0.0f;
+        if (GEO == 0) {
+            ap_i = (OUTER_TX ? A.arx : A.atx)[(int64_t)i * A.np + p];
+        } else {
+            bwd_elem<GEO>(A, !OUTER_TX, i, p, pv, tau_i, ap_i);
+        }
         const float w = OUTER_TX ? ap_o * ap_i : ap_i * ap_o;
         if (!(fabsf(w) >= A.thresh)) continue;
-        const float tau_i = ti[(int64_t)i * A.np + p];
+        if (GEO == 0) tau_i = (OUTER_TX ? A.trx : A.ttx)[(int64_t)i * A.np + p];
         const float x = OUTER_TX ? tau_o + tau_i : tau_i + tau_o;
         const int tx = OUTER_TX ? o : i, rx = OUTER_TX ? i : o;
         double q = 0.0;
@@ -184,8 +220,37 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
             ga = __builtin_fmaf(g * ap_i, v, ga);
         }
     }
-    if (A.gtau) A.gtau[(int64_t)o * A.np + p] = gt;
-    if (A.gapod) A.gapod[(int64_t)o * A.np + p] = ga;
+    if (act && A.gtau) A.gtau[(int64_t)o * A.np + p] = gt;
+    if (act && A.gapod) A.gapod[(int64_t)o * A.np + p] = ga;
+    if (GEO > 0 && A.gc0part) {
+        // fixed-order workgroup sum in double: wave butterflies, then the 4 waves
+        double c = (act && gt != 0.0f) ? (double)gt * (-(double)tau_o / (double)A.c0) : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+        __shared__ double wsum[4];
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            A.gc0part[A.gc0_off + (int64_t)o * gridDim.x + blockIdx.x] =
+                (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    }
+}
+
+// d loss / d c0 from the partials of both delay-gradient launches, summed in a
+// fixed order (one workgroup): strided per-lane sums, then a tree
+__global__ __launch_bounds__(256) void k_gc0_sum(const double *__restrict__ part, int64_t n,
+                                                 float *__restrict__ gc0)
+{
+    __shared__ double t[256];
+    double c = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) c += part[i];
+    t[threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) t[threadIdx.x] += t[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *gc0 = (float)t[0];
 }
 
 // max |value| of gout, apodtx and apodrx (blockIdx.y = 0, 1, 2), as bit
@@ -196,6 +261,11 @@ __global__ __launch_bounds__(256) void k_bwd_scale(BwdArgs A, int64_t ngout)
 {
     const int which = blockIdx.y;
     const float *x = which == 0 ? A.gout : (which == 1 ? A.atx : A.arx);
+    if (!x) {
+        // synthetic geometry: its weights are 0 or 1
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(A.mbits + which, __float_as_uint(1.0f));
+        return;
+    }
     const int64_t n = which == 0 ? ngout : (int64_t)(which == 1 ? A.ntx : A.nrx) * A.np;
     auto fold = [](float m, float v) {
         v = fabsf(v);
@@ -248,7 +318,7 @@ __device__ __forceinline__ double bwd_scale(const unsigned *mbits, int per_bin =
 // LDS bins are 64-bit fixed-point integers (ds_add_u64): on gfx950 integer
 // LDS atomics run at about 37x the rate of float LDS atomics, and integer
 // sums do not depend on the order of the additions.
-template <int GK>
+template <int GK, int GEO>
 __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
 {
     __shared__ unsigned long long by[BWD_BINS];
@@ -262,8 +332,10 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
     const int nent = A.ntx + A.nrx;
     const float *bt = A.bounds + (tile * nent + tx) * 2;
     const float txlo = bt[0], txhi = bt[1];
-    const float ttx = act ? A.ttx[(int64_t)tx * A.np + p] : 0.0f;
-    const float atx = act ? A.atx[(int64_t)tx * A.np + p] : 0.0f;
+    float pv[3];
+    bwd_pixel<GEO>(A, act ? p : 0, pv);
+    float ttx = 0.0f, atx = 0.0f;
+    if (act) bwd_elem<GEO>(A, true, tx, p, pv, ttx, atx);
     const double scale = bwd_scale(A.mbits);
     const bool fix = scale > 0.0;  // uniform
     const double iscale = fix ? 1.0 / scale : 0.0;
@@ -315,10 +387,13 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
         float *gy = A.gy + chan * A.gstride;
         float *ge = (GK == GK_CUBIC) ? A.ge + chan * A.gstride : nullptr;
         if (act) {
-            const float arx = A.arx[(int64_t)rx * A.np + p];
+            float trx = 0.0f, arx = 0.0f;
+            if (GEO == 0) arx = A.arx[(int64_t)rx * A.np + p];
+            else bwd_elem<GEO>(A, false, rx, p, pv, trx, arx);
             const float w = atx * arx;
             if (fabsf(w) >= A.thresh) {
-                const float x = ttx + A.trx[(int64_t)rx * A.np + p];
+                if (GEO == 0) trx = A.trx[(int64_t)rx * A.np + p];
+                const float x = ttx + trx;
                 const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)] * w;
                 if (GK == GK_NEAREST) {
                     int it = 0;
@@ -400,7 +475,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
 // the order of the pixels.  16 waves per workgroup sweep 1024 pixels per
 // step; a wave skips its 64 pixels when the tile's delay range leaves the
 // channel unused.
-template <int GK>
+template <int GK, int GEO>
 __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
 {
     extern __shared__ unsigned long long bins[];
@@ -433,8 +508,6 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
     for (int t = tid; t < narr * nb; t += blockDim.x) bins[t] = 0ull;
     __syncthreads();
 
-    const float *ttx = A.ttx + (int64_t)tx * A.np, *atx = A.atx + (int64_t)tx * A.np;
-    const float *trx = A.trx + (int64_t)rx * A.np, *arx = A.arx + (int64_t)rx * A.np;
     for (int64_t base = 0; base < A.np; base += blockDim.x) {
         const int64_t p = base + tid;
         // the wave's pixels lie in one tile of 256: skip it when the tile's
@@ -446,9 +519,24 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         const float xlo = bt[0] + br[0], xhi = bt[1] + br[1];
         if (!(xlo <= xhi)) continue;
         if (p >= A.np) continue;
-        const float w = atx[p] * arx[p];
+        float ttx = 0.0f, atx = 0.0f, trx = 0.0f, arx = 0.0f;
+        if (GEO == 0) {
+            // tables: the delays are read only where the weight passes
+            atx = A.atx[(int64_t)tx * A.np + p];
+            arx = A.arx[(int64_t)rx * A.np + p];
+        } else {
+            float pv[3];
+            bwd_pixel<GEO>(A, p, pv);
+            bwd_elem<GEO>(A, true, tx, p, pv, ttx, atx);
+            bwd_elem<GEO>(A, false, rx, p, pv, trx, arx);
+        }
+        const float w = atx * arx;
         if (!(fabsf(w) >= A.thresh)) continue;
-        const float x = ttx[p] + trx[p];
+        if (GEO == 0) {
+            ttx = A.ttx[(int64_t)tx * A.np + p];
+            trx = A.trx[(int64_t)rx * A.np + p];
+        }
+        const float x = ttx + trx;
         const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)] * w;
         if (g == 0.0f) continue;
         if (GK == GK_NEAREST) {
@@ -1001,26 +1089,36 @@ static int fill_zero(float *p, int64_t n, hipStream_t s)
     return check_launch("fill zero");
 }
 
-template <int GK>
+template <int GK, int GEO>
 static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
-                        float *gapodrx, float *grf, hipStream_t s)
+                        float *gapodrx, float *grf, float *gc0, hipStream_t s)
 {
     const unsigned ptiles = (unsigned)((a.np + 255) / 256);
-    if (gtautx || gapodtx) {
+    const bool want_c0 = GEO > 0 && gc0 && a.gc0part;
+    if (gtautx || gapodtx || want_c0) {
         BwdArgs b = a;
         b.gtau = gtautx;
         b.gapod = gapodtx;
+        b.gc0part = want_c0 ? a.gc0part : nullptr;
+        b.gc0_off = 0;
         ProfScope ps("das_bwd_tau", s);
-        hipLaunchKernelGGL((k_das_bwd_tau<GK, 1>), dim3(ptiles, a.ntx), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 1, GEO>), dim3(ptiles, a.ntx), dim3(256), 0, s, b);
         if (int rc = check_launch("das_bwd_tau")) return rc;
     }
-    if (gtaurx || gapodrx) {
+    if (gtaurx || gapodrx || want_c0) {
         BwdArgs b = a;
         b.gtau = gtaurx;
         b.gapod = gapodrx;
+        b.gc0part = want_c0 ? a.gc0part : nullptr;
+        b.gc0_off = (int)(ptiles * (unsigned)a.ntx);
         ProfScope ps("das_bwd_tau", s);
-        hipLaunchKernelGGL((k_das_bwd_tau<GK, 0>), dim3(ptiles, a.nrx), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 0, GEO>), dim3(ptiles, a.nrx), dim3(256), 0, s, b);
         if (int rc = check_launch("das_bwd_tau")) return rc;
+    }
+    if (want_c0) {
+        hipLaunchKernelGGL(k_gc0_sum, dim3(1), dim3(256), 0, s, (const double *)a.gc0part,
+                           (int64_t)ptiles * (a.ntx + a.nrx), gc0);
+        if (int rc = check_launch("gc0_sum")) return rc;
     }
     if (grf) {
         const int64_t nch = (int64_t)a.nfr * a.ntx * a.nrx;
@@ -1047,7 +1145,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         hipLaunchKernelGGL(k_bwd_scale, dim3(sb, 3), dim3(256), 0, s, a, ngout);
         if (int rc = check_launch("das_bwd_scale")) return rc;
         if (per_channel) {
-            hipLaunchKernelGGL(k_das_bwd_rf_ch<GK>, dim3((unsigned)nch), dim3(1024), ch_lds, s, a);
+            hipLaunchKernelGGL((k_das_bwd_rf_ch<GK, GEO>), dim3((unsigned)nch), dim3(1024), ch_lds, s,
+                               a);
         } else {
             if (GK == GK_CUBIC || GK == GK_KORDER || GK == GK_AKIMA) {
                 if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
@@ -1061,7 +1160,8 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
                 // the cotangent rows (grf itself, or the upsampled grid's rows)
                 if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
             }
-            hipLaunchKernelGGL(k_das_bwd_rf<GK>, dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_das_bwd_rf<GK, GEO>), dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s,
+                               a);
         }
         if (int rc = check_launch("das_bwd_rf")) return rc;
         if (GK == GK_CUBIC) {
@@ -1164,16 +1264,30 @@ int launch_upsample_adjoint(const float *gu, int64_t nch, int32_t nt, int32_t us
     return check_launch("spline_adjoint");
 }
 
-int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
-                   float *gapodrx, float *grf, hipStream_t s)
+template <int GEO>
+static int launch_das_bwd_geo(int gk, const BwdArgs &a, float *gtautx, float *gapodtx,
+                              float *gtaurx, float *gapodrx, float *grf, float *gc0, hipStream_t s)
 {
     switch (gk) {
-    case GK_NEAREST: return launch_bwd_t<GK_NEAREST>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
-    case GK_LINEAR: return launch_bwd_t<GK_LINEAR>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
-    case GK_KORDER: return launch_bwd_t<GK_KORDER>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
-    case GK_AKIMA: return launch_bwd_t<GK_AKIMA>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
-    default: return launch_bwd_t<GK_CUBIC>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, s);
+    case GK_NEAREST:
+        return launch_bwd_t<GK_NEAREST, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_LINEAR:
+        return launch_bwd_t<GK_LINEAR, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_KORDER:
+        return launch_bwd_t<GK_KORDER, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_AKIMA:
+        return launch_bwd_t<GK_AKIMA, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    default:
+        return launch_bwd_t<GK_CUBIC, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
     }
+}
+
+int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
+                   float *gapodrx, float *grf, float *gc0, hipStream_t s)
+{
+    if (a.geo == 2) return launch_das_bwd_geo<2>(gk, a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    if (a.geo == 3) return launch_das_bwd_geo<3>(gk, a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    return launch_das_bwd_geo<0>(gk, a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
 }
 
 }  // namespace pycbf

@@ -113,6 +113,9 @@ _SIGS = {
     "pycbf_das_tabbed_bwd": ([_prf, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, ct.c_int,
                               ct.c_int, _vp, _f32, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _sz, _vp], ct.c_int),
+    "pycbf_das_synthetic_bwd": ([_prf, _vp, _i64, _vp, _vp, _f32, _vp, _i64, ct.c_int, ct.c_int,
+                                 _vp, _f32, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _sz, _vp], ct.c_int),
     "pycbf_ray_delays": ([_vp, ct.c_int, ct.c_int, _f32, _f32, _f32, _f32, _vp, _vp, ct.c_int,
                           _f32, _vp, ct.c_int, _vp, _i64, ct.c_int, _vp, _vp, _vp, _vp, _vp],
                          ct.c_int),

@@ -7,6 +7,9 @@ Akima / makima coefficients).
 
 The forward is pycbf's Tabbed DAS (pycbf/cpu/pycbf.c:179-271 with the GPU
 layouts of pycbf/gpu/__engines__.cu:813-864); pycbf itself has no gradients.
+das_synthetic does the same for the Synthetic DAS (pycbf/gpu/__engines__.cu:
+382-470, 620-694) through pycbf_das_synthetic / pycbf_das_synthetic_bwd,
+with gradients for the channel data and the speed of sound and no delay table.
 torch provides the device memory, the stream and the autograd graph only.
 """
 from __future__ import annotations
@@ -177,18 +180,104 @@ def das_tabbed(rf, tautx, apodtx, taurx, apodrx, t0, dt, interp="cubic", sumtype
     return out[0] if squeeze else out
 
 
+def _synthetic_function():
+    torch = _torch()
+
+    class SyntheticDASFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, rf, c0, bf):
+            nfr = rf.shape[0]
+            info = bf._rfinfo()
+            kind = bf.interp["kind"]
+            need = E.lib.pycbf_das_workspace_bytes(ctypes.byref(info), nfr, bf.nop,
+                                                   E.INTERP[kind], bf._interp_param(), 1)
+            ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
+            out = torch.empty(bf._out_shape(nfr), dtype=torch.float32, device=rf.device)
+            c0f = float(c0)
+            rc = E.lib.pycbf_das_synthetic(
+                ctypes.byref(info), E.ptr(rf), nfr, E.ptr(bf._txpack), E.ptr(bf._rxpack), c0f,
+                E.ptr(bf._pnts), E.ptr(bf._perm), bf.nop, E.INTERP[kind], bf._interp_param(),
+                E.ptr(bf._S_dev()), float(bf.thresh), E.SUMTYPE[bf.sumtype], E.ptr(out),
+                E.ptr(ws), ws.numel(), E.stream_of(rf.device))
+            E.check(rc, "pycbf_das_synthetic")
+            ctx.save_for_backward(rf)
+            ctx.bf = bf
+            ctx.c0 = c0f
+            ctx.c0_shape = c0.shape if torch.is_tensor(c0) else None
+            return out
+
+        @staticmethod
+        def backward(ctx, gout):
+            (rf,) = ctx.saved_tensors
+            bf = ctx.bf
+            need_rf, need_c0 = ctx.needs_input_grad[:2]
+            gout = gout.to(torch.float32).contiguous()
+            nfr = rf.shape[0]
+            grf = torch.empty_like(rf) if need_rf else None
+            gc0 = torch.empty(1, dtype=torch.float32, device=rf.device) if need_c0 else None
+            if grf is None and gc0 is None:
+                return None, None, None
+            info = bf._rfinfo()
+            kind = bf.interp["kind"]
+            need = E.lib.pycbf_das_bwd_workspace_bytes(ctypes.byref(info), nfr, bf.nop,
+                                                       E.INTERP[kind], bf._interp_param())
+            ws = torch.empty(max(int(need), 1), dtype=torch.uint8, device=rf.device)
+            rc = E.lib.pycbf_das_synthetic_bwd(
+                ctypes.byref(info), E.ptr(rf), nfr, E.ptr(bf._txpack), E.ptr(bf._rxpack),
+                ctx.c0, E.ptr(bf._pnts), bf.nop, E.INTERP[kind], bf._interp_param(),
+                E.ptr(bf._S_dev()), float(bf.thresh), E.SUMTYPE[bf.sumtype], E.ptr(gout),
+                E.ptr(grf), None, None, None, None, E.ptr(gc0), E.ptr(ws), ws.numel(),
+                E.stream_of(rf.device))
+            E.check(rc, "pycbf_das_synthetic_bwd")
+            if gc0 is not None:
+                gc0 = gc0.reshape(ctx.c0_shape)
+            return grf, gc0, None
+
+    return SyntheticDASFunction
+
+
+_SFN = None
+
+
+def das_synthetic(rf, bf, c0=None):
+    """Differentiable forward of a SyntheticBeamformer ``bf`` (its geometry,
+    interpolation, sumtype and thresh) with respect to the channel data and
+    the speed of sound, with no delay or apodisation table anywhere: the
+    forward is pycbf_das_synthetic, the backward pycbf_das_synthetic_bwd,
+    whose kernels re-evaluate the geometry per tile in registers.  ``c0`` is
+    a 0-dim float tensor (set requires_grad for d loss / d c0) or None for
+    ``bf.c0``.  The gradient takes every delay as a path length over c0, the
+    transmit time intercepts t0tx included, as pycbf's helpers build them
+    (t0tx = distance / c): the derivative of the image of the geometry
+    rebuilt at another sound speed.  rf is
+    (ntx, nrx, nt) or (nfr, ntx, nrx, nt) float32 on the beamformer's device;
+    the output has the beamformer's frame layout."""
+    global _SFN
+    torch = _torch()
+    if _SFN is None:
+        _SFN = _synthetic_function()
+    if not getattr(bf, "_synthetic", False):
+        raise BeamformerException("das_synthetic needs a SyntheticBeamformer")
+    squeeze = rf.dim() == 3
+    rf4 = rf[None] if squeeze else rf
+    rf4 = _check(rf4, "rf", (rf4.shape[0], bf.ntx, bf.nrx, bf.nt))
+    if c0 is None:
+        c0 = torch.tensor(float(bf.c0), dtype=torch.float32)
+    out = _SFN.apply(rf4, c0, bf)
+    return out[0] if squeeze else out
+
+
 def das_beamformer(rf, bf, tables=None):
     """Differentiable forward of a TabbedBeamformer or SyntheticBeamformer
     ``bf`` with respect to the channel data (and, for tables passed in, the
-    tables): its interpolation, sumtype and thresh, through das_tabbed.  A
-    SyntheticBeamformer's geometry is evaluated once into device tables
-    (``bf.tables()``: pycbf_synthetic_tables, the float32 operations its
-    gather evaluates per sample), so the geometry itself is a constant.
-    ``tables`` (tautx, apodtx, taurx, apodrx) overrides the beamformer's."""
+    tables): its interpolation, sumtype and thresh.  A SyntheticBeamformer
+    without ``tables`` goes through das_synthetic (geometry re-evaluated in
+    registers, forward and backward, no table stored); a TabbedBeamformer, or
+    ``tables`` (tautx, apodtx, taurx, apodrx) given, through das_tabbed."""
     torch = _torch()
     if tables is None:
-        if hasattr(bf, "tables"):
-            tables = bf.tables()
+        if getattr(bf, "_synthetic", False):
+            return das_synthetic(rf, bf)
         else:
             tables = tuple(torch.as_tensor(getattr(bf, n), dtype=torch.float32,
                                            device=rf.device).contiguous()

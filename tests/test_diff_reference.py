@@ -32,6 +32,23 @@ def test_natural_y2_matches_scipy():
     np.testing.assert_allclose(got, ref, atol=1e-9)
 
 
+def test_natural_y2_sweep_matches_scipy():
+    """The tridiagonal sweep used for large inputs without autograd."""
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(2)
+    y = rng.standard_normal((3, 700))
+    dt = 0.7
+    rhs = (y[:, 2:] - 2.0 * y[:, 1:-1] + y[:, :-2]) * (6.0 / dt ** 2)
+    got = diff64._solve_141(torch.as_tensor(rhs))
+    for c in range(3):
+        ref = CubicSpline(np.arange(700) * dt, y[c], bc_type="natural")(np.arange(700) * dt, 2)
+        np.testing.assert_allclose(got[c].numpy(), ref[1:-1], atol=1e-9)
+    big = torch.as_tensor(rng.standard_normal((1100, 4000)))
+    sweep = diff64.natural_y2(big, dt)  # above the size where the sweep is taken
+    dense = diff64.natural_y2(big[:2].clone().requires_grad_(), dt).detach()
+    np.testing.assert_allclose(sweep[:2].numpy(), dense.numpy(), atol=1e-9)
+
+
 @pytest.mark.parametrize("kind", ["nearest", "linear", "cubic", "korder_cubic"])
 def test_forward_agrees_with_float32_oracle(kind):
     from oracle import numerics
