@@ -292,8 +292,10 @@ def demod_leg(dev, lines, dz, reps, with_cpu):
     """Demodulation of the coherence leg's per-receive images to IQ along
     depth (the notebooks' fft_tukey_demod, 5-12 MHz band at 1540 m/s): every
     line of 2048 axial samples of all receive channels and columns, one
-    float32-accurate product per call (three-way BF16 split on the BF16
-    matrix units).  Returns the JSON object of the leg."""
+    workgroup per line running the real FFT, band, window, shift and the
+    Bluestein inverse DFT in LDS (pycbf_demod_fft); the dense operator product
+    (three-way BF16 split on the matrix units, pycbf_demod_apply) is timed
+    beside it.  Returns the JSON object of the leg."""
     import torch
 
     from pycbf_amd.gpu import _engine as E
@@ -301,34 +303,45 @@ def demod_leg(dev, lines, dz, reps, with_cpu):
     kmin, kmax, ks = 2 * 5e6 / 1540.0, 2 * 12e6 / 1540.0, 1.0 / dz
     rows, n = lines.shape
     ikmin, ikmax = band(n, kmin, kmax, ks)
-    ncol = 2 * (ikmax - ikmin)
-    iq = fft_tukey_demod(lines, kmin, kmax, ks)
-    torch.cuda.synchronize()
-    E.lib.pycbf_profile_reset()
-    E.lib.pycbf_profile_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        iq = fft_tukey_demod(lines, kmin, kmax, ks)
-    torch.cuda.synchronize()
-    call_ms = (time.perf_counter() - t0) * 1e3 / reps
-    E.lib.pycbf_profile_enable(0)
-    k_ms, k_n = E.profile_read("demod_gemm")
-    kms = k_ms / max(k_n, 1)
-    flops = 2.0 * rows * n * ncol          # the float32 product
-    tf = flops / (kms * 1e-3) / 1e12
-    tf_bf16 = 6.0 * tf                       # six BF16 partial products per term
-    log(f"demod: {rows} lines x {n} -> {ncol // 2} IQ, kernel {kms * 1e3:.1f} us "
-        f"({tf:.1f} TFLOP/s float32-equivalent, {tf_bf16:.0f} TFLOP/s on the BF16 units), "
-        f"call {call_ms:.3f} ms")
+    m = ikmax - ikmin
+
+    def timed(method, kname):
+        iq = fft_tukey_demod(lines, kmin, kmax, ks, method=method)
+        torch.cuda.synchronize()
+        E.lib.pycbf_profile_reset()
+        E.lib.pycbf_profile_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            iq = fft_tukey_demod(lines, kmin, kmax, ks, method=method)
+        torch.cuda.synchronize()
+        call = (time.perf_counter() - t0) * 1e3 / reps
+        E.lib.pycbf_profile_enable(0)
+        k_ms, k_n = E.profile_read(kname)
+        return iq, k_ms / max(k_n, 1), call
+
+    iq, kms, call_ms = timed("fft", "demod_fft")
+    iq_op, op_ms, _ = timed("operator", "demod_gemm")
+    # compulsory HBM bytes: every line read once, its IQ written once (the
+    # tables, a few hundred KB, stay in cache)
+    hbm = 4.0 * rows * n + 8.0 * rows * m
+    achieved = hbm / (kms * 1e-3) / 1e9
+    op_err = float((iq_op - iq).abs().max() / iq.abs().max())
+    log(f"demod: {rows} lines x {n} -> {m} IQ, FFT kernel {kms * 1e3:.1f} us "
+        f"({achieved:.0f} GB/s, {achieved / HBM_PEAK_GBPS:.3f} of HBM), operator product "
+        f"{op_ms * 1e3:.1f} us (max difference {op_err:.2e} of peak), call {call_ms:.3f} ms")
     res = {
         "workload": f"fft_tukey_demod of {rows} lines x {n} axial samples (dz {dz * 1e6:.2f} um, "
-                    f"5-12 MHz) to {ncol // 2} IQ samples",
+                    f"5-12 MHz) to {m} IQ samples",
         "lines_per_s": round(rows / (kms * 1e-3), 1),
+        "kernel": "demod_fft",
         "kernel_us": round(kms * 1e3, 2),
         "call_ms": round(call_ms, 3),
-        "fp32_equiv_tflops": round(tf, 2),
-        "roofline": {"bound": "mfma", "achieved": round(tf_bf16, 1), "peak": BF16_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(tf_bf16 / BF16_MFMA_PEAK_TFLOPS, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "bytes": "lines read once (float32) + IQ written once (complex64)"},
+        "operator_product": {"kernel": "demod_gemm", "kernel_us": round(op_ms * 1e3, 2),
+                             "speedup_fft_vs_operator": round(op_ms / kms, 3),
+                             "max_diff_rel_peak_vs_fft": float(f"{op_err:.3g}")},
     }
     if with_cpu:
         from oracle import demod as odemod
@@ -499,12 +512,71 @@ def adjoint_leg(dev, reps):
         f"das_bwd_rf {kern['das_bwd_rf']:.3f} ms")
     del rf, tabs
     torch.cuda.empty_cache()
+    syn = synthetic_adjoint_leg(dev, reps, ntx, nrx, nt, side)
     return {
         "workload": f"cubic Tabbed DAS {ntx} PW x {nrx} rx x {nt} samples onto {side}x{side} px, "
                     "forward + backward with respect to the channel data",
         "ms_fwd_bwd": round(wall * 1e3, 3),
         "fwd_bwd_gsamples_per_s": round(samples / wall / 1e9, 2),
         "kernel_ms": kern,
+        "synthetic": syn,
+    }
+
+
+def synthetic_adjoint_leg(dev, reps, ntx, nrx, nt, side):
+    """The same shape as a Synthetic DAS (plane waves as synthetic points):
+    forward + backward with respect to the channel data and c0 through
+    das_synthetic (geometry re-evaluated per tile in registers, no table),
+    against das_tabbed on the beamformer's stored tables (evaluated once,
+    outside the timed step).  Returns the JSON object."""
+    import torch
+
+    from pycbf_amd import sim
+    from pycbf_amd.gpu import SyntheticBeamformer
+    from pycbf_amd.gpu.autograd import das_synthetic, das_tabbed
+    xele = sim.linear_array(nrx)
+    steers = np.radians(np.linspace(-15, 15, ntx))
+    _, _, pnts = sim.grid(side, side, (-19e-3, 19e-3), (5e-3, 45e-3))
+    bf = SyntheticBeamformer(**sim.pw_synthetic(steers, xele, pnts), t0=0.0, dt=1.0 / sim.FS,
+                             nt=nt, interp={"kind": "cubic"}, sumtype="tx_and_rx", device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(28)
+    rf = torch.randn((1, ntx, nrx, nt), generator=g, device=dev, requires_grad=True)
+    G = torch.randn((1, side * side), generator=g, device=dev)
+    c0 = torch.tensor(float(bf.c0), dtype=torch.float32, requires_grad=True)
+    tabs = bf.tables()
+    table_bytes = sum(t.numel() * 4 for t in tabs)
+
+    def step_syn():
+        out = das_synthetic(rf, bf, c0)
+        return torch.autograd.grad((out * G).sum(), [rf, c0])
+
+    def step_tab():
+        out = das_tabbed(rf, *tabs, 0.0, 1.0 / sim.FS, "cubic", "tx_and_rx", float(bf.thresh))
+        return torch.autograd.grad((out * G).sum(), [rf])
+
+    res = {}
+    for name, fn in (("registers", step_syn), ("stored_tables", step_tab)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / reps * 1e3
+    log(f"synthetic adjoint: fwd+bwd (rf, c0) with geometry in registers {res['registers']:.3f} "
+        f"ms; with stored tables (rf) {res['stored_tables']:.3f} ms; tables "
+        f"{table_bytes / 1e6:.1f} MB")
+    del rf, tabs, bf
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"cubic Synthetic DAS {ntx} PW (synthetic points) x {nrx} rx x {nt} samples "
+                    f"onto {side}x{side} px, forward + backward",
+        "ms_fwd_bwd_registers": round(res["registers"], 3),
+        "ms_fwd_bwd_stored_tables": round(res["stored_tables"], 3),
+        "table_bytes_avoided": int(table_bytes),
+        "registers_grads": "channel data and c0",
+        "stored_tables_grads": "channel data",
     }
 
 

@@ -212,6 +212,26 @@ int pycbf_make_tabs_2d(int mode, int64_t ntx, const double *txc, int64_t nrx, co
 int pycbf_demod_apply(const float *rf, int64_t rows, int64_t n, const float *opT, int64_t ncol,
                       float *iq, void *workspace, size_t workspace_bytes, void *stream);
 
+/* The same demodulation by FFTs, for n a power of two in 8..4096: one
+ * workgroup per line (no reference counterpart beyond the notebook's NumPy
+ * chain rfft -> band -> Tukey window -> ifftshift -> ifft, which it follows).
+ * Per line: the n-point real FFT (an n/2-point complex FFT and the real
+ * split), the nbin kept bins from ikmin (bin ikmin + (k + shift) mod nbin
+ * at position k: shift = nbin / 2 for the ifftshift, 0 without), and their
+ * nbin-point inverse DFT by Bluestein's chirp product over a circular
+ * convolution of nconv points (a power of two >= 2 nbin - 1, <= 2048).
+ * tables (device, float32 pairs, built by the host in float64):
+ *   pre[nbin]   window[(k + shift) mod nbin] * exp(i pi k^2 / nbin)
+ *   post[nbin]  exp(i pi t^2 / nbin) / (nbin * nconv)
+ *   bhat[nconv] FFT of the chirp b[d] = exp(-i pi d^2 / nbin), |d| < nbin,
+ *               placed at d mod nconv
+ *   split[nbin] exp(-2 pi i kb / n) of the bin kb = ikmin + (k + shift) mod nbin
+ *   tw[T]       exp(-2 pi i t / T), T = max(n / 2, nconv)
+ * rf (rows, n) and iq (rows, 2 nbin) as pycbf_demod_apply (Re, Im pairs);
+ * the line is read once and its IQ written once.  8-byte aligned pointers. */
+int pycbf_demod_fft(const float *rf, int64_t rows, int64_t n, int64_t ikmin, int64_t nbin,
+                    int64_t shift, int64_t nconv, const float *tables, float *iq, void *stream);
+
 /* Bytes of device workspace pycbf_demod_apply needs for rows lines of n
  * samples and ncol outputs: 0 when the output tiles fill the GPU, else the
  * partial products of a split over the samples (few lines, long n).  No

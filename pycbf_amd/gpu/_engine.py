@@ -99,6 +99,7 @@ _SIGS = {
                             ct.c_double, _vp, _vp, _vp, _vp, _vp], ct.c_int),
     "pycbf_demod_apply": ([_vp, _i64, _i64, _vp, _i64, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_demod_workspace_bytes": ([_i64, _i64, _i64], _sz),
+    "pycbf_demod_fft": ([_vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp], ct.c_int),
     "pycbf_spline_coefs": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_y2": ([_vp, _i64, _i32, _f32, _vp, _vp, _sz, _vp], ct.c_int),
     "pycbf_spline_workspace_bytes": ([_i64, _i32], _sz),

@@ -61,22 +61,28 @@ def _lines(nlines, n, dz, seed):
     return x + 0.01 * rng.standard_normal(x.shape)
 
 
-@pytest.mark.parametrize("n,shiftit", [(1024, True), (777, True), (512, False)])
-def test_demod_matches_oracle(n, shiftit):
+@pytest.mark.parametrize("n,shiftit,method,dz", [
+    (1024, True, None, 20e-6), (777, True, None, 20e-6), (512, False, None, 20e-6),
+    (1024, True, "operator", 20e-6), (2048, True, "fft", 9.625e-6),
+    (4096, False, "fft", 5e-6), (4096, True, "fft", 20e-6), (8192, True, None, 20e-6), (64, True, "fft", 20e-6)])
+def test_demod_matches_oracle(n, shiftit, method, dz):
+    """Both methods (the FFT kernel for power-of-two lengths, the operator
+    product otherwise or on request) against the FP64 NumPy chain."""
     import torch
     from pycbf_amd.gpu.demod import fft_tukey_demod
-    print(f"demod lines n={n} shiftit={shiftit}", flush=True)
-    dz = 20e-6
+    print(f"demod lines n={n} shiftit={shiftit} method={method}", flush=True)
     kmin, kmax, ks = 2 * 5e6 / 1540, 2 * 12e6 / 1540, 1 / dz
     x = _lines(96, n, dz, n)
     want = od.fft_tukey_demod(x, kmin, kmax, ks, shiftit=shiftit)
     got = fft_tukey_demod(torch.as_tensor(x.astype(np.float32), device="cuda"), kmin, kmax, ks,
-                          shiftit=shiftit)
+                          shiftit=shiftit, method=method)
     assert got.dtype == torch.complex64 and tuple(got.shape) == want.shape
     g = got.cpu().numpy()
     peak = np.abs(want).max()
     assert peak > 0
-    # float32 input rounding plus float32 accumulation over n samples
+    # float32 input rounding plus float32 accumulation over n samples (or
+    # over the log2 n stages of the transforms)
+    print(f"  max error {np.abs(g - want).max() / peak:.2e} of peak", flush=True)
     assert np.abs(g - want).max() <= 2e-5 * peak
 
 
