@@ -30,6 +30,9 @@ if ROOT not in sys.path:
 METRIC = "Gsamples/s (tx·rx·px) cubic DAS + SLSC px/s on 1 MI355X vs plain GPU & CPU"
 HBM_PEAK_GBPS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+# float32 wave64 VALU instructions: one per CU per cycle (a 16-lane SIMD takes
+# 4 cycles per instruction, 4 SIMDs per CU), 256 CUs at 2.4 GHz
+VALU_PEAK_GINSTR = 256 * 2.4
 
 NTX, NRX, NT, NX, NZ = 75, 128, 4096, 512, 512
 
@@ -63,21 +66,24 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
                                "das_gather_synthetic_cubic_75x128x512x512_nt4096_16fr.md")
 
 
-def measured_traffic(frames):
-    """HBM-side bytes per gather launch from the committed rocprofv3 summary
-    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size), when it
-    was measured for this frame count; None otherwise."""
+def profile_figures(frames):
+    """Per-launch figures of the gather from the committed rocprofv3 summary,
+    when it was measured for this frame count: {"traffic": HBM-side bytes
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size),
+    "valu": wave-level VALU instructions (SQ_INSTS_VALU)}; empty otherwise."""
     if not os.path.exists(TRAFFIC_PROFILE):
-        return None
+        return {}
     vals = {}
+    keys = ("traffic_bytes_per_launch", "valu_instructions_per_launch", "frames")
     with open(TRAFFIC_PROFILE) as fh:
         for line in fh:
-            if ":" in line and line.split(":")[0] in ("traffic_bytes_per_launch", "frames"):
+            if ":" in line and line.split(":")[0] in keys:
                 k, v = line.split(":", 1)
                 vals[k] = float(v)
-    if vals.get("frames") != frames or "traffic_bytes_per_launch" not in vals:
-        return None
-    return vals["traffic_bytes_per_launch"]
+    if vals.get("frames") != frames:
+        return {}
+    return {"traffic": vals.get("traffic_bytes_per_launch"),
+            "valu": vals.get("valu_instructions_per_launch")}
 
 
 def cpu_info():
@@ -783,7 +789,8 @@ def main():
     table_bytes = 8.0 * args.frames * NTX * NRX * (NT + 1)
     compulsory = table_bytes + 12.0 * npx + 4.0 * npx * args.frames
     hbm_achieved = compulsory / (gather_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.frames)
+    prof = profile_figures(args.frames)
+    traffic = prof.get("traffic")
     # on-chip: the stencil reads (16 B per contributing sample and frame) the
     # gather serves from LDS, against the measured LDS read bandwidth
     contrib = count_contributing(bf, 1e-2)
@@ -864,6 +871,15 @@ def main():
                             "bytes": "16 B per contributing sample and frame",
                             "contributing_samples_per_frame": contrib,
                             "all_samples_per_frame": NTX * NRX * npx},
+        "roofline_issue": None if not prof.get("valu") else {
+            "bound": "valu", "unit": "Ginstr/s",
+            "achieved": round(prof["valu"] / (gather_ms * 1e-3) / 1e9, 1),
+            "peak": VALU_PEAK_GINSTR,
+            "frac": round(prof["valu"] / (gather_ms * 1e-3) / 1e9 / VALU_PEAK_GINSTR, 4),
+            "instructions": prof["valu"],
+            "bytes": "wave-level VALU instructions per launch (SQ_INSTS_VALU, committed "
+                     "profile) over this run's gather time; peak one float32 wave64 VALU "
+                     "instruction per CU per cycle at the 2.4 GHz peak clock"},
         "peaks_measured_gbps": peaks,
         "plain_gpu": None if plain_ms is None else {
             "value": round(samples / (plain_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
