@@ -199,6 +199,32 @@ def time_variant(env, value, params, rf, ref_out, steps):
     return ms, err
 
 
+def time_tabbed(bf, rf, ref_out, steps):
+    """Times the same 16-frame step through a TabbedBeamformer on the
+    geometry's own delay / apodisation tables (pycbf's table-driven class);
+    returns (ms per step, max error relative to the headline images' peak)."""
+    import torch
+
+    from pycbf_amd.gpu import TabbedBeamformer
+    tt, at, tr, ar = bf.tables()
+    bft = TabbedBeamformer(tautx=tt, apodtx=at, taurx=tr, apodrx=ar, t0=0.0, dt=1.0 / 40e6,
+                           nt=NT, interp={"kind": "cubic"}, sumtype="tx_and_rx",
+                           device=rf.device)
+    del tt, at, tr, ar
+    outv = torch.empty_like(ref_out)
+    bft(rf, out_as_numpy=False, buffer=outv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bft(rf, out_as_numpy=False, buffer=outv)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    err = float((outv - ref_out).abs().max() / ref_out.abs().max())
+    del bft, outv
+    torch.cuda.empty_cache()
+    return ms, err
+
+
 SLSC_NXO, SLSC_NZO, SLSC_SUB, SLSC_LAGS, SLSC_KWIN = 256, 256, 8, list(range(1, 26)), 8
 
 
@@ -797,7 +823,7 @@ def main():
     lds_bytes = 16.0 * contrib * args.frames
     lds_achieved = lds_bytes / (gather_ms * 1e-3) / 1e9
     peaks = {"hbm": None, "l2": None, "lds": None}
-    plain_ms = plain_err = direct_ms = direct_err = None
+    plain_ms = plain_err = direct_ms = direct_err = tab_ms = tab_err = None
     if not args.no_compare:
         peaks = {k: round(E.measure_bandwidth(k, dev), 1) for k in ("hbm", "l2", "lds")}
         log(f"measured read bandwidth GB/s: {peaks}; contributing samples per frame {contrib} "
@@ -805,6 +831,8 @@ def main():
         # plain-GPU comparators on the same workload and data
         plain_ms, plain_err = time_variant("PYCBF_DAS_PLAIN", "1", params, rf, out, 2)
         direct_ms, direct_err = time_variant("PYCBF_DAS_LDS", "0", params, rf, out, 2)
+        tab_ms, tab_err = time_tabbed(bf, rf, out, 3)
+        log(f"TabbedBeamformer on the same tables {tab_ms:.2f} ms/step (err {tab_err:.2e})")
         log(f"thread-per-sample comparator {plain_ms:.2f} ms/step (err {plain_err:.2e}); direct "
             f"gather {direct_ms:.2f} ms/step (err {direct_err:.2e})")
 
@@ -887,6 +915,12 @@ def main():
             "max_err_rel_peak_vs_headline": float(f"{plain_err:.3g}"),
             "kernel": "thread per (frame, tx, rx, px) sample with one float atomic each: "
                       "pycbf's GPU launch scheme (PYCBF_DAS_PLAIN=1)"},
+        "tabbed_gpu": None if tab_ms is None else {
+            "value": round(samples / (tab_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
+            "ms_per_step": round(tab_ms, 3),
+            "max_err_rel_peak_vs_headline": float(f"{tab_err:.3g}"),
+            "kernel": "TabbedBeamformer over the geometry's delay / apodisation tables "
+                      "(table-derived pixel order)"},
         "direct_gpu": None if direct_ms is None else {
             "value": round(samples / (direct_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
             "ms_per_step": round(direct_ms, 3),

@@ -198,18 +198,59 @@ class TabbedBeamformer(Tabbed, GPUBeamformer):
         self._taurx = self._to_dev(self.taurx)
         self._apodtx = self._to_dev(self.apodtx)
         self._apodrx = self._to_dev(self.apodrx)
+        # the gather runs over copies of the tables whose pixel columns are in
+        # a compact processing order (contiguous loads per wave); its image
+        # comes back in that order and is scattered to the pixels' own order
         self._perm = None
+        self._order = self._to_dev(self._delay_order(), dtype=_torch().int64)
+        self._otabs = tuple(t.index_select(1, self._order).contiguous()
+                            for t in (self._tautx, self._apodtx, self._taurx, self._apodrx))
+        self._obuf = None
+
+    def _delay_order(self):
+        """Processing order of the pixels from the tables alone (the tables
+        carry no pixel positions): each pixel's delays to the first and last
+        receive elements (transmits when there is one receive element) act as
+        two-centre coordinates, and pycbf_tile_order's Morton order of them
+        makes every 256-pixel tile a compact patch, whose channels' delay
+        windows stay narrow.  Only the grouping of pixels into tiles and the
+        table layout change; each pixel's sums are the same."""
+        torch = _torch()
+
+        def row(t, i):
+            r = t[i] if isinstance(t, torch.Tensor) else np.asarray(t)[i]
+            r = r.detach().cpu().numpy() if isinstance(r, torch.Tensor) else r
+            return np.asarray(r, np.float64)
+
+        if self.nrx >= 2:
+            rows = [row(self.taurx, 0), row(self.taurx, self.nrx - 1)]
+        elif self.ntx >= 2:
+            rows = [row(self.tautx, 0), row(self.tautx, self.ntx - 1)]
+        else:
+            rows = [row(self.taurx, 0)]
+        co = np.stack(rows, axis=1)
+        finite = np.isfinite(co)
+        big = np.abs(co[finite]).max() * 2.0 + 1.0 if finite.any() else 1.0
+        co = np.ascontiguousarray(np.where(finite, co, big), dtype=np.float32)
+        perm = np.empty(self.nop, np.int32)
+        E.check(E.lib.pycbf_tile_order(E.ptr(co), self.nop, co.shape[1], E.ptr(perm)),
+                "pycbf_tile_order")
+        return perm
 
     def _run(self, rf, nframes, pout):
         info = self._rfinfo()
         ws = self._workspace(nframes)
         kind = self.interp["kind"]
+        if self._obuf is None or self._obuf.shape != pout.shape:
+            self._obuf = _torch().empty_like(pout)
+        tt, at, tr, ar = self._otabs
         rc = E.lib.pycbf_das_tabbed(
-            ct_byref(info), E.ptr(rf), nframes, E.ptr(self._tautx), E.ptr(self._apodtx),
-            E.ptr(self._taurx), E.ptr(self._apodrx), E.ptr(self._perm), self.nop,
-            E.INTERP[kind], self._interp_param(), E.ptr(self._S_dev()), float(self.thresh),
-            E.SUMTYPE[self.sumtype], E.ptr(pout), E.ptr(ws), ws.numel(), E.stream_of(self._dev))
+            ct_byref(info), E.ptr(rf), nframes, E.ptr(tt), E.ptr(at), E.ptr(tr), E.ptr(ar),
+            None, self.nop, E.INTERP[kind], self._interp_param(), E.ptr(self._S_dev()),
+            float(self.thresh), E.SUMTYPE[self.sumtype], E.ptr(self._obuf), E.ptr(ws),
+            ws.numel(), E.stream_of(self._dev))
         E.check(rc, "pycbf_das_tabbed")
+        pout.index_copy_(pout.dim() - 1, self._order, self._obuf)
 
 
 @dataclass(kw_only=True)
