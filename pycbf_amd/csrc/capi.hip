@@ -6,6 +6,9 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -210,6 +213,7 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
     a.c0 = geo.c0;
     a.ttx_lo = ttx_lo;
     a.trx_lo = trx_lo;
+    a.rc0 = geo.ndim > 0 ? 1.0f / geo.c0 : 0.0f;
     int rc = 0;
     switch (kind) {
     case PYCBF_INTERP_NEAREST:
@@ -276,6 +280,32 @@ static int run_das(const pycbf_rfinfo *info, const float *rf, int64_t nfr, const
         return launch_das_lds(plan.gk, a, plan.nsplit, part, s);
     }
     return launch_das(plan.gk, a, plan.nsplit, part, s);
+}
+
+bool recip_division_exact(float c)
+{
+    if (!(fabsf(c) >= 0x1p-30f && fabsf(c) <= 0x1p+30f)) return false;
+    static std::mutex mu;
+    static std::vector<std::pair<float, bool>> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto &e : cache)
+            if (e.first == c) return e.second;
+    }
+    const float r = 1.0f / c;
+    bool ok = true;
+    // every significand of [1, 2); rounding to nearest is symmetric in the sign
+    for (uint32_t m = 0; m < (1u << 23) && ok; ++m) {
+        float x;
+        const uint32_t bits = 0x3f800000u | m;
+        memcpy(&x, &bits, 4);
+        const float q = x * r;
+        const float e = std::fma(-q, c, x);
+        ok = (std::fma(e, r, q) == x / c);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    cache.emplace_back(c, ok);
+    return ok;
 }
 
 static float acos_threshold(float ala)

@@ -90,7 +90,16 @@ struct DasArgs {
     // direct gather only: float32 delay residuals (may be null); samples are
     // placed at (ttx + ttx_lo) + (trx + trx_lo) in double, exact weights
     const float *ttx_lo, *trx_lo;
+    float rc0;  // 1 / c0 rounded (rdt above is 1 / dt rounded)
 };
+
+// True when, for every float x with 2^-60 <= |x| < 2^60,
+//   q = x r, e = fma(-q, c, x), fma(e, r, q)   (r = 1 / c rounded)
+// is the correctly rounded x / c (c itself within 2^-30 .. 2^30).  Decided
+// by trying every significand of one binade (in that range no step
+// underflows or overflows, so the steps scale exactly by powers of two);
+// cached per divisor.  Host only.
+bool recip_division_exact(float c);
 
 int launch_das(int gk, const DasArgs &a, int nsplit, float *partial, hipStream_t s);
 // LDS-staged gather (delay tables or in-register synthetic geometry); bounds

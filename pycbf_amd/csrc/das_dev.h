@@ -7,6 +7,20 @@
 
 namespace pycbf {
 
+// x / c for a divisor that recip_division_exact(c) accepts (r = 1 / c
+// rounded): the product with the reciprocal and one fused correction, the
+// correctly rounded quotient for 2^-60 <= |x| < 2^60.  Its callers' dividends
+// lie there or do not matter: a knot offset x - t0 in [0, tmax - t0]
+// truncates to index 0 when tiny; a path length of 0 gives a delay of 0 (its
+// sign may differ, which changes no sum); an overflowing length gives NaN
+// instead of infinity (out of the trace either way).
+__device__ __forceinline__ float div_rn(float x, float c, float r)
+{
+    const float q = x * r;
+    const float e = __builtin_fmaf(-q, c, x);
+    return __builtin_fmaf(e, r, q);
+}
+
 // Knot index of pycbf/cpu/pycbf.c:75-87: fill outside [t0, tmax] (NaN too).
 // The clamp to the last interval only matters when t0 is so large that tmax
 // rounds more than one sample past the end (the reference then reads out of
@@ -15,6 +29,16 @@ __device__ __forceinline__ bool knot_index(float x, float t0, float dt, float tm
 {
     if (!(x >= t0 && x <= tmax)) return false;
     j = min((int)((x - t0) / dt), n - 1);
+    return true;
+}
+
+// knot_index with the quotient by div_rn (rdt = 1 / dt rounded, dt accepted
+// by recip_division_exact): the same index
+__device__ __forceinline__ bool knot_index_rn(float x, float t0, float dt, float rdt, float tmax,
+                                              int n, int &j)
+{
+    if (!(x >= t0 && x <= tmax)) return false;
+    j = min((int)div_rn(x - t0, dt, rdt), n - 1);
     return true;
 }
 
@@ -63,10 +87,13 @@ __device__ __forceinline__ bool acos_gt(float v, float T) { return (v >= -1.0f &
 // "acos(v) <= ala"
 __device__ __forceinline__ bool acos_le(float v, float T) { return (v >= -1.0f && v <= 1.0f) && !(v < T); }
 
-template <int NDIM>
+// FD: quotients by c0 through div_rn (rc0 = 1 / c0 rounded, c0 accepted by
+// recip_division_exact); the same delays
+template <int NDIM, bool FD = false>
 __device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, const float *p,
-                                        float &tau, float &apod)
+                                        float &tau, float &apod, float rc0 = 0.0f)
 {
+    auto over_c0 = [&](float v) { return FD ? div_rn(v, c0, rc0) : v / c0; };
     float mag = 0.0f, proj = 0.0f;
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
@@ -80,7 +107,7 @@ __device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, 
     const float T = pk[8];
     tau = 0.0f;
     if (mode == 0) {  // plane wave
-        tau = proj / c0;
+        tau = over_c0(proj);
         apod = 1.0f;
         return;
     }
@@ -89,12 +116,12 @@ __device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, 
         const bool in_cone = ((double)fabsf(proj) > 1e-12) && ((double)mag > 1e-12) &&
                              acos_le(fabsf(proj / mag), T);
         if (in_cone) {
-            tau = sgn * (mag / c0) + t0;
+            tau = sgn * over_c0(mag) + t0;
             apod = 1.0f;
         } else {
             const float lat = sqrtf(fabsf(mag * mag - proj * proj));
             if ((double)lat < (double)pk[9] / 2.0) {
-                tau = proj * pk[10] / c0 + t0;
+                tau = over_c0(proj * pk[10]) + t0;
                 apod = 1.0f;
             } else {
                 apod = 0.0f;
@@ -102,13 +129,13 @@ __device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, 
         }
         return;
     }
-    tau = ((double)fabsf(proj) > 1e-12) ? sgn * (mag / c0) + t0 : t0;
+    tau = ((double)fabsf(proj) > 1e-12) ? sgn * over_c0(mag) + t0 : t0;
     apod = (((double)mag > 1e-12) && acos_gt(fabsf(proj / mag), T)) ? 0.0f : 1.0f;
 }
 
-template <int NDIM>
+template <int NDIM, bool FD = false>
 __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, const float *p,
-                                        float &tau, float &apod)
+                                        float &tau, float &apod, float rc0 = 0.0f)
 {
     float mag = 0.0f, proj = 0.0f;
 #pragma unroll
@@ -118,7 +145,7 @@ __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, 
         proj += di * pk[3 + d];
     }
     mag = sqrtf(mag);
-    tau = mag / c0;
+    tau = FD ? div_rn(mag, c0, rc0) : mag / c0;
     apod = (((double)mag > 1e-12) && acos_gt(proj / mag, pk[6])) ? 0.0f : 1.0f;
 }
 
@@ -126,7 +153,8 @@ __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, 
 // gather + interpolate for FC frames of one channel
 // ---------------------------------------------------------------------------
 // delay and weight of element e of the tx (or rx) axis at pixel p
-template <int GEO>
+// FD: quotients by c0 through div_rn (see geom_tx)
+template <int GEO, bool FD = false>
 __device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int64_t p,
                                           const float (&pv)[3], float &tau, float &apod)
 {
@@ -135,9 +163,9 @@ __device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int6
         tau = tx ? A.ttx[o] : A.trx[o];
         apod = tx ? A.atx[o] : A.arx[o];
     } else if (tx) {
-        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
+        geom_tx<GEO, FD>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
     } else {
-        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
+        geom_rx<GEO, FD>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
     }
 }
 

@@ -388,7 +388,11 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
 // element one pass computes every inner channel's window into an LDS table;
 // channel b + 1 is in flight while b is evaluated (two LDS buffers, one
 // barrier per staged channel).  A window wider than the slot takes direct
-// global gathers, as in k_das_lds.
+// global gathers, as in k_das_lds.  The quotients by c0 (geometry) and dt
+// (knot index) are products with the rounded reciprocals plus one fused
+// correction (div_rn), which the launch takes only for divisors whose every
+// quotient it rounds like the division (recip_division_exact): the same
+// delays and indices with three VALU instructions instead of a division.
 // ---------------------------------------------------------------------------
 constexpr int DMA_MAXB = 256;    // inner channels per outer element (window table)
 constexpr int DMA_CHUNKS = 768;  // 16-byte chunks per buffer (12 KB), all frames
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
 
     for (int a = a0; a < a1; ++a) {
         float ta = 0.0f, wa = 0.0f;
-        if (live) elem_eval<GEO>(A, OUTER_TX, a, p, pv, ta, wa);
+        if (live) elem_eval<GEO, true>(A, OUTER_TX, a, p, pv, ta, wa);
         const float amin = bnd[2 * (boff_a + a)], amax = bnd[2 * (boff_a + a) + 1];
         for (int b = tid; b < nb; b += LDS_BLOCK) {
             int lo, cnt;
@@ -489,7 +493,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             const int2 w = wins[b];
             if (w.y == 0) return;
             float tb = 0.0f, wb = 0.0f;
-            if (live) elem_eval<GEO>(A, !OUTER_TX, b, p, pv, tb, wb);
+            if (live) elem_eval<GEO, true>(A, !OUTER_TX, b, p, pv, tb, wb);
             const float wgt = OUTER_TX ? wa * wb : wb * wa;  // apodtx * apodrx
             const float x = OUTER_TX ? ta + tb : tb + ta;    // tautx + taurx
             if (!(live && fabsf(wgt) >= A.thresh)) return;
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
                 return;
             }
             int j = 0;
-            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j)) return;
+            if (!knot_index_rn(x, A.t0, A.dt, A.rdt, A.tmax, A.nsrc, j)) return;
             const float xlead = A.t0 + A.dt * (float)(j + 1);
             const float xlag = A.t0 + A.dt * (float)j;
             float ka = (xlead - x) * A.rdt;
@@ -661,7 +665,10 @@ int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStr
     const bool dma = gk == GK_CUBIC && a.sumflag != PYCBF_SUM_NONE && (fc == 4 || fc == 16) &&
                      (a.sumflag == PYCBF_SUM_TX_AND_RX ? nb : (a.sumflag == PYCBF_SUM_TX_ONLY
                                                                    ? a.ntx : a.nrx)) <= DMA_MAXB &&
-                     das_cubic_dma_enabled();
+                     das_cubic_dma_enabled() &&
+                     // its quotients by dt and c0 are products with the reciprocals
+                     a.rdt == 1.0f / a.dt && recip_division_exact(a.dt) &&
+                     (a.geo == 0 || (a.rc0 == 1.0f / a.c0 && recip_division_exact(a.c0)));
     if (dma) {
         ProfScope prof("das_gather", s);
         if (fc == 16)
