@@ -251,3 +251,38 @@ def test_synthetic_sumtypes_and_frames():
                                        p["nvecrx"], p["alarx"], p["c0"], p["pnts"],
                                        kind="cubic", sumtype=sumtype)
             assert_close_to_peak(got[f], ref, 1e-5, f"{sumtype} frame {f}")
+
+
+@pytest.mark.parametrize("sumtype", ["tx_and_rx", "none"])
+def test_tabbed_processing_order_keeps_every_pixel(sumtype):
+    """TabbedBeamformer groups pixels into tiles by a Morton code of two of
+    their receive delays and keeps its tables in that order; its images (every
+    output at its own pixel) equal the engine's on the tables in the pixels'
+    own order (no processing order).  Sums agree to float rounding: a pixel's
+    window may be staged in one grouping and gathered directly in the other."""
+    import torch
+    from pycbf_amd.gpu import _engine as E
+    print(f"tabbed processing order, {sumtype}", flush=True)
+    w = small_tabbed(seed=4, ntx=7, nrx=48, nt=1536, nx=48, nz=96)
+    rf = np.stack([w["rf"], 0.7 * w["rf"]]).astype(np.float32)
+    bf = _tabbed(w, {"kind": "cubic"}, sumtype)
+    assert not torch.equal(bf._order.cpu(), torch.arange(bf.nop))
+    got = np.asarray(bf(rf))
+    dev = torch.device("cuda", 0)
+    # (tautx, apodtx, taurx, apodrx) in the pixels' own order
+    tabs = [torch.as_tensor(np.asarray(t, np.float32), device=dev).contiguous()
+            for t in w["tabs"]]
+    rfd = torch.as_tensor(rf, device=dev)
+    info = bf._rfinfo()
+    kind = E.INTERP["cubic"]
+    need = E.lib.pycbf_das_workspace_bytes(ctypes.byref(info), 2, bf.nop, kind, 1, 0)
+    ws = torch.empty(int(need), dtype=torch.uint8, device=dev)
+    ref = torch.empty(bf._out_shape(2), dtype=torch.float32, device=dev)
+    rc = E.lib.pycbf_das_tabbed(ctypes.byref(info), E.ptr(rfd), 2, *(E.ptr(t) for t in tabs),
+                                None, bf.nop, kind, 1, None, float(bf.thresh),
+                                E.SUMTYPE[sumtype], E.ptr(ref), E.ptr(ws), ws.numel(),
+                                E.stream_of(dev))
+    E.check(rc, "pycbf_das_tabbed")
+    ref = ref.cpu().numpy().reshape(got.shape)
+    assert np.isfinite(got).all()
+    assert_close_to_peak(got, ref, 1e-6, f"processing order {sumtype}")
