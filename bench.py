@@ -396,11 +396,14 @@ def sos_leg(dev, steps):
     """Differentiable workload: 31 plane waves over +-15 deg x 128 rx x 4096
     samples, 256 x 256 pixels, a 64 x 64 sound-speed map (1450 m/s above
     10 mm, 1540 m/s below, +-20 m/s blobs); channel data simulated through the
-    true map with the straight-ray model; `steps` Adam steps on the 64 x 64
-    map from a uniform 1540 m/s start (lr 2 m/s, a light smoothness penalty),
-    each a forward and a backward pass through the ray-delay and DAS kernels;
-    then the same from a 4 x 4 grid of speeds bilinearly upsampled to the map
-    (a labelled variant).  Returns the JSON object of the leg."""
+    true map with the straight-ray model.  From a uniform 1540 m/s start the
+    64 x 64 map is fitted to the cross-angle consistency of the per-transmit
+    images (SoSProblem.angle_loss, lag 3): `steps` / 4 Adam steps on a depth
+    profile (one speed per map row, lr 5 m/s), then 3 `steps` / 4 on the free
+    map (lr 2 m/s, smoothness 1e-3), each a forward and a backward pass through
+    the ray-delay and DAS kernels.  Labelled variants: the image-focus loss on
+    the free map and on a 4 x 4 grid of speeds, 50 steps each.  Returns the JSON
+    object of the leg."""
     import torch
 
     from pycbf_amd import sim
@@ -419,12 +422,14 @@ def sos_leg(dev, steps):
                       dt=1.0 / sim.FS)
     ct = f(ctrue)
     c0 = torch.full_like(ct, 1540.0)
+    top = int(round((10e-3 - geo.zm0) / geo.dzm))      # map rows above the 10 mm layer boundary
     with torch.no_grad():
         e0 = float((prob.images(c0).sum(0) ** 2).mean())
-        loss_true = float(prob.loss(ct, e0))
-    top = int(round((10e-3 - geo.zm0) / geo.dzm))      # map rows above the 10 mm layer boundary
+        angle_true = float(prob.angle_loss(ct))
+        angle_start = float(prob.angle_loss(c0))
+        focus_true = float(prob.loss(ct, e0))
 
-    def optimise(theta, cmap, lr, smooth, timed):
+    def optimise(theta, cmap, lossfn, lr, smooth, nsteps, timed=False):
         prob.smooth_weight = smooth
         opt = torch.optim.Adam([theta], lr=lr)
         hist = []
@@ -433,10 +438,10 @@ def sos_leg(dev, steps):
             E.lib.pycbf_profile_enable(1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(nsteps):
             opt.zero_grad()
             c = cmap()
-            loss = prob.loss(c, e0)
+            loss = lossfn(c)
             loss.backward()
             opt.step()
             # kept on the device: reading here would stall the host every step
@@ -447,55 +452,73 @@ def sos_leg(dev, steps):
         if timed:
             E.lib.pycbf_profile_enable(0)
         prob.smooth_weight = 0.0
-        cfin = cmap().detach()
-        with torch.no_grad():
-            lfin = float(prob.loss(cfin, e0))
-        h = torch.stack(hist).cpu().numpy()
-        return wall, h, cfin, lfin
+        return wall, torch.stack(hist).cpu().numpy(), cmap().detach()
 
-    free = c0.clone().requires_grad_()
-    wall, hist, cfin, lfin = optimise(free, lambda: free, 2.0, 1e-4, True)
+    angle = lambda c: prob.angle_loss(c, 3)  # noqa: E731
+    nprof = max(1, steps // 4)
+    nfree = max(1, steps - nprof)
+    prof = torch.full((w["nzm"], 1), 1540.0, device=dev, requires_grad=True)
+    _, hist_p, cprof = optimise(prof, lambda: prof.expand(w["nzm"], w["nxm"]), angle, 5.0, 0.0,
+                                nprof)
+    free = cprof.clone().requires_grad_()
+    wall, hist_f, cfin = optimise(free, lambda: free, angle, 2.0, 1e-3, nfree, timed=True)
     kern = {}
     for k in ("ray_delays", "ray_delays_bwd", "das_gather", "das_bwd_tau", "spline_coefs"):
         ms, n = E.profile_read(k)
         kern[k] = round(ms / max(n, 1), 4)
-    rmse0 = float(torch.sqrt(((c0 - ct) ** 2).mean()))
-    rmse1 = float(torch.sqrt(((cfin - ct) ** 2).mean()))
-    top1, toptrue = float(cfin[:top].mean()), float(ct[:top].mean())
+    with torch.no_grad():
+        angle_fin = float(prob.angle_loss(cfin))
+    rmse = lambda c: float(torch.sqrt(((c - ct) ** 2).mean()))  # noqa: E731
+    layers = lambda c: (round(float(c[:top].mean()), 2), round(float(c[top:].mean()), 2))  # noqa: E731
+
+    # labelled variants: the image-focus loss (free map; 4 x 4 grid of speeds)
+    focus = lambda c: prob.loss(c, e0)  # noqa: E731
+    ffree = c0.clone().requires_grad_()
+    _, hist_ff, cff = optimise(ffree, lambda: ffree, focus, 2.0, 1e-4, 50)
     theta = torch.full((1, 1, 4, 4), 1540.0, device=dev, requires_grad=True)
 
     def coarse():
         return torch.nn.functional.interpolate(theta, size=(w["nzm"], w["nxm"]), mode="bilinear",
                                                align_corners=True)[0, 0]
 
-    _, hist4, c4, l4 = optimise(theta, coarse, 10.0, 0.0, False)
+    _, hist4, c4 = optimise(theta, coarse, focus, 10.0, 0.0, 50)
     ntx, nrx = len(w["steers"]), len(w["xele"])
     samples = ntx * nrx * w["gridshape"][0] * w["gridshape"][1]
-    log(f"sos leg: {steps} Adam steps on the 64x64 map {wall * 1e3 / steps:.2f} ms/step, loss "
-        f"{hist[0, 0]:.5f} -> {lfin:.5f} (true map {loss_true:.5f}), map rmse {rmse0:.2f} -> "
-        f"{rmse1:.2f} m/s, top layer 1540 -> {top1:.1f} (true {toptrue:.1f}) m/s; 4x4 variant "
-        f"loss -> {l4:.5f}, rmse -> {float(torch.sqrt(((c4 - ct) ** 2).mean())):.2f}")
+    log(f"sos leg: cross-angle fit, {nprof} depth-profile + {nfree} free-map Adam steps "
+        f"({wall * 1e3 / nfree:.2f} ms/step): loss {angle_start:.4f} -> {angle_fin:.4f} (true map "
+        f"{angle_true:.4f}), map rmse {rmse(c0):.2f} -> {rmse(cprof):.2f} -> {rmse(cfin):.2f} m/s, "
+        f"layers {layers(c0)} -> {layers(cfin)} (true {layers(ct)}) m/s; focus loss, free map: "
+        f"rmse -> {rmse(cff):.2f}; 4x4: rmse -> {rmse(c4):.2f}")
     del rf, prob
     torch.cuda.empty_cache()
+    r3 = lambda a: [round(float(v), 3) for v in a]  # noqa: E731
+    r5 = lambda a: [round(float(v), 5) for v in a]  # noqa: E731
     return {
         "workload": f"differentiable DAS for speed of sound: {ntx} PW x {nrx} rx x {w['nt']} "
                     f"samples, {w['gridshape'][0]}x{w['gridshape'][1]} px, "
-                    f"{w['nzm']}x{w['nxm']} map optimised directly (Adam lr 2 m/s, smoothness "
-                    f"1e-4), straight-ray delays with float32 residuals, cubic, {steps} Adam steps",
-        "ms_per_step": round(wall * 1e3 / steps, 3),
-        "fwd_bwd_gsamples_per_s": round(samples / (wall / steps) / 1e9, 2),
-        "loss_per_step": [round(float(v), 5) for v in hist[:, 0]],
-        "map_rmse_per_step_m_per_s": [round(float(v), 3) for v in hist[:, 1]],
-        "loss_final": round(lfin, 6), "loss_at_true_map": round(loss_true, 6),
-        "loss_below_true_map": bool(lfin < loss_true),
-        "map_rmse_m_per_s": [round(rmse0, 3), round(rmse1, 3)],
-        "top_layer_mean_m_per_s": {"start": 1540.0, "end": round(top1, 2),
-                                   "true": round(toptrue, 2)},
+                    f"{w['nzm']}x{w['nxm']} map from 1540 m/s fitted to the cross-angle "
+                    f"consistency of the per-transmit images (lag 3): {nprof} Adam steps on a "
+                    f"depth profile (lr 5 m/s), {nfree} on the free map (lr 2 m/s, smoothness "
+                    f"1e-3); straight-ray delays with float32 residuals, cubic",
+        "ms_per_step": round(wall * 1e3 / nfree, 3),
+        "fwd_bwd_gsamples_per_s": round(samples / (wall / nfree) / 1e9, 2),
+        "loss_per_step": r5(np.concatenate([hist_p[:, 0], hist_f[:, 0]])),
+        "map_rmse_per_step_m_per_s": r3(np.concatenate([hist_p[:, 1], hist_f[:, 1]])),
+        "loss_start": round(angle_start, 6), "loss_final": round(angle_fin, 6),
+        "loss_at_true_map": round(angle_true, 6),
+        "map_rmse_m_per_s": {"start": round(rmse(c0), 3), "after_profile": round(rmse(cprof), 3),
+                             "final": round(rmse(cfin), 3)},
+        "layer_means_m_per_s": {"start": layers(c0), "final": layers(cfin), "true": layers(ct),
+                                "rows": "above / below 10 mm"},
         "kernel_ms": kern,
-        "variant_4x4_grid": {"lr": 10.0, "loss_final": round(l4, 6),
-                             "map_rmse_m_per_s": [round(rmse0, 3),
-                                                  round(float(torch.sqrt(((c4 - ct) ** 2).mean())), 3)],
-                             "loss_per_step": [round(float(v), 5) for v in hist4[::5, 0]]},
+        "variant_focus_loss_free_map": {
+            "lr": 2.0, "smoothness": 1e-4, "steps": 50, "loss_at_true_map": round(focus_true, 6),
+            "loss_final": round(float(hist_ff[-1, 0]), 6),
+            "map_rmse_m_per_s": [round(rmse(c0), 3), round(rmse(cff), 3)],
+            "loss_per_step": r5(hist_ff[:, 0])},
+        "variant_focus_loss_4x4_grid": {
+            "lr": 10.0, "steps": 50, "map_rmse_m_per_s": [round(rmse(c0), 3), round(rmse(c4), 3)],
+            "loss_per_step": r5(hist4[::5, 0])},
     }
 
 
@@ -732,7 +755,7 @@ def main():
                     help="frames timed for the CPU baseline (about 10-30 s of host work)")
     ap.add_argument("--no-slsc", action="store_true", help="skip the coherence leg")
     ap.add_argument("--no-sos", action="store_true", help="skip the differentiable leg")
-    ap.add_argument("--sos-steps", type=int, default=50)
+    ap.add_argument("--sos-steps", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the ragged-batch leg")
     ap.add_argument("--batch-runs", type=int, default=5)
     ap.add_argument("--batch-speckle", type=int, default=10_000)

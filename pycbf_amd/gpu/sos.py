@@ -1,10 +1,13 @@
 """Speed-of-sound estimation through the differentiable DAS.
 
 The chain slowness map -> straight-ray delays (pycbf_ray_delays, adjoint
-pycbf_ray_delays_bwd) -> per-receive-channel DAS images (pycbf_das_tabbed,
-adjoint pycbf_das_tabbed_bwd) -> a focus loss is differentiated end to end:
-each stage is a torch.autograd.Function over hand-written kernels, and only
-the scalar loss on the images is written with torch operations.
+pycbf_ray_delays_bwd) -> per-receive-channel or per-transmit DAS images
+(pycbf_das_tabbed_residual, adjoint pycbf_das_tabbed_bwd) -> a scalar loss
+is differentiated end to end: each stage is a torch.autograd.Function over
+hand-written kernels, and only the scalar loss on the images is written with
+torch operations.  Two losses: the cross-angle consistency of the
+per-transmit images (SoSProblem.angle_loss) and an image-focus loss (lag-one
+coherence and energy of the per-receive images, SoSProblem.loss).
 
 The reference corrects delays with an external eikonal solver
 (examples/rough_sos_correction.ipynb); this module states its own
@@ -177,6 +180,35 @@ class SoSProblem:
                           "tx_only", 0.0, tautx_lo=tlo, taurx_lo=rlo)
         nx, nz = self.gridshape
         return imrx.reshape(-1, nx, nz)
+
+    def tx_images(self, cmap):
+        """Per-transmit images (ntx, nx, nz) for the sound-speed map cmap
+        (nzm, nxm) in m/s (differentiable): each plane wave's image summed
+        over the receive elements."""
+        slow = 1.0 / cmap
+        tautx, taurx, tlo, rlo = ray_delays(slow, self.geo, self.steer, self.xorig, self.xele,
+                                            self.pvec, self.c0, self.nsteps, residuals=True)
+        im = das_tabbed(self.rf, tautx, self._ones(tautx), taurx, self._ones(taurx), self.t0,
+                        self.dt, self.interp, "rx_only", 0.0, tautx_lo=tlo, taurx_lo=rlo)
+        nx, nz = self.gridshape
+        return im.reshape(-1, nx, nz)
+
+    def angle_loss(self, cmap, lag=3):
+        """Cross-angle consistency of the map: the energy of the differences
+        between the images of plane waves `lag` steering steps apart, relative
+        to the images' energy, plus smooth_weight times the mean squared
+        neighbour difference of the map in m/s.  With the right delays every
+        plane wave images the scatterers at the same place, so the
+        differences shrink (the principle of computed ultrasound tomography
+        in echo mode).  Kernels in float32, the scalar loss in float64."""
+        im = self.tx_images(cmap).double()
+        d = im[lag:] - im[:-lag]
+        loss = (d * d).sum() / (im * im).sum()
+        if self.smooth_weight > 0.0:
+            c = cmap.double()
+            rough = ((c[1:] - c[:-1]) ** 2).mean() + ((c[:, 1:] - c[:, :-1]) ** 2).mean()
+            loss = loss + self.smooth_weight * rough
+        return loss
 
     def _ones(self, like):
         torch = _torch()
