@@ -36,11 +36,15 @@ struct RegTB {
     static constexpr int value = SUM == PYCBF_SUM_TX_ONLY ? 16 : 8;
 };
 
-// natural-cubic value of the table row `row` at x (0 outside the trace)
+// natural-cubic value of the table row `row` at x (0 outside the trace); FD:
+// the knot quotient by dt through div_rn (dt accepted by recip_division_exact)
+template <bool FD>
 __device__ __forceinline__ float cubic_row(const float *__restrict__ row, float x, const DasArgs &A)
 {
     int j = 0;
-    if (!knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j)) return 0.0f;
+    const bool in = FD ? knot_index_rn(x, A.t0, A.dt, A.rdt, A.tmax, A.nsrc, j)
+                       : knot_index(x, A.t0, A.dt, A.tmax, A.nsrc, j);
+    if (!in) return 0.0f;
     const float xlead = A.t0 + A.dt * (float)(j + 1);
     const float xlag = A.t0 + A.dt * (float)j;
     const float a = (xlead - x) * A.rdt;
@@ -62,7 +66,9 @@ __global__ __launch_bounds__(256) void k_fill(float *__restrict__ p, int64_t n, 
         p[i] = v;
 }
 
-template <int GEO, int SUM>
+// FD: quotients by dt and c0 through div_rn (both accepted by
+// recip_division_exact; the same delays and indices as the divisions)
+template <int GEO, int SUM, bool FD>
 __global__ __launch_bounds__(256) void k_das_reg(DasArgs A, int64_t ntiles, int rchunk)
 {
     constexpr int REG_TB = RegTB<SUM>::value;
@@ -89,13 +95,14 @@ __global__ __launch_bounds__(256) void k_das_reg(DasArgs A, int64_t ntiles, int 
         for (int k = 0; k < REG_TB; ++k) {
             ttx[k] = 0.0f;
             atx[k] = 0.0f;
-            if (act && t0 + k < A.ntx) elem_eval<GEO>(A, true, t0 + k, p, pv, ttx[k], atx[k]);
+            if (act && t0 + k < A.ntx)
+                elem_eval<GEO, FD>(A, true, t0 + k, p, pv, ttx[k], atx[k]);
         }
 #pragma unroll 1
         for (int ii = 0; ii < nrc; ++ii) {
             const int r = r0 + ii;
             float trx = 0.0f, arx = 0.0f;
-            if (act) elem_eval<GEO>(A, false, r, p, pv, trx, arx);
+            if (act) elem_eval<GEO, FD>(A, false, r, p, pv, trx, arx);
             const float *base = A.src + ((int64_t)t0 * A.nrx + r) * rowlen;
             float part = 0.0f;
 #pragma unroll
@@ -104,7 +111,8 @@ __global__ __launch_bounds__(256) void k_das_reg(DasArgs A, int64_t ntiles, int 
                 const float w = atx[k] * arx;          // apodtx * apodrx
                 if (act && fabsf(w) >= A.thresh)
                     part = __builtin_fmaf(
-                        w, cubic_row(base + (int64_t)k * A.nrx * rowlen, ttx[k] + trx, A), part);
+                        w, cubic_row<FD>(base + (int64_t)k * A.nrx * rowlen, ttx[k] + trx, A),
+                        part);
             }
             if (SUM == PYCBF_SUM_TX_AND_RX) {
                 acc += part;
@@ -128,15 +136,26 @@ bool das_reg_applies(int gk, int64_t nfr, int sumflag)
            (sumflag == PYCBF_SUM_TX_AND_RX || sumflag == PYCBF_SUM_TX_ONLY);
 }
 
+template <int SUM, bool FD>
+static void reg_geo_fd(const DasArgs &a, dim3 grid, int64_t ntiles, int rchunk, hipStream_t s)
+{
+    if (a.geo == 3)
+        hipLaunchKernelGGL((k_das_reg<3, SUM, FD>), grid, dim3(256), 0, s, a, ntiles, rchunk);
+    else if (a.geo == 2)
+        hipLaunchKernelGGL((k_das_reg<2, SUM, FD>), grid, dim3(256), 0, s, a, ntiles, rchunk);
+    else
+        hipLaunchKernelGGL((k_das_reg<0, SUM, FD>), grid, dim3(256), 0, s, a, ntiles, rchunk);
+}
+
 template <int SUM>
 static void reg_geo(const DasArgs &a, dim3 grid, int64_t ntiles, int rchunk, hipStream_t s)
 {
-    if (a.geo == 3)
-        hipLaunchKernelGGL((k_das_reg<3, SUM>), grid, dim3(256), 0, s, a, ntiles, rchunk);
-    else if (a.geo == 2)
-        hipLaunchKernelGGL((k_das_reg<2, SUM>), grid, dim3(256), 0, s, a, ntiles, rchunk);
+    const bool fd = a.rdt == 1.0f / a.dt && recip_division_exact(a.dt) &&
+                    (a.geo == 0 || (a.rc0 == 1.0f / a.c0 && recip_division_exact(a.c0)));
+    if (fd)
+        reg_geo_fd<SUM, true>(a, grid, ntiles, rchunk, s);
     else
-        hipLaunchKernelGGL((k_das_reg<0, SUM>), grid, dim3(256), 0, s, a, ntiles, rchunk);
+        reg_geo_fd<SUM, false>(a, grid, ntiles, rchunk, s);
 }
 
 int launch_das_reg(const DasArgs &a, hipStream_t s)

@@ -3,9 +3,9 @@
 // samples, the m kept bins [ikmin, ikmin + m) weighted by the Tukey window
 // and rotated by the ifftshift, and the m-point inverse DFT of those bins.
 //
-// One wave per line (up to four per workgroup, which sweeps the lines), everything
-// in LDS, one read of the line and one write of its IQ samples (the
-// operation's compulsory HBM bytes):
+// One wave per line (up to four per workgroup, which sweeps the lines),
+// everything in LDS, one read of the line and one write of its IQ samples
+// (the operation's compulsory HBM bytes):
 //   1. z_j = x_2j + i x_2j+1 (n/2 complex points), Z = FFT_{n/2}(z) and the
 //      real-FFT split X_k = (Z_k + conj Z_{n/2-k}) / 2
 //                         - i/2 e^{-2 pi i k / n} (Z_k - conj Z_{n/2-k});
@@ -21,11 +21,12 @@
 // The host precomputes, in float64 rounded to float32: pre_k = w_src(k)
 // e^{i pi k^2 / m}, post_t = e^{i pi t^2 / m} / (m L), the chirp transform,
 // the split twiddle e^{-2 pi i kb / n} of each kept bin, and one twiddle
-// table e^{-2 pi i t / T} (T = max(n / 2, L)) that serves every FFT size.  The FFTs are Stockham autosort transforms (N.
-// Govindaraju et al., "High performance discrete Fourier transforms on
-// graphics processors", SC 2008) in radix 16 (a radix-2 and a radix-4 stage
-// first where log2 N needs them), in place in the wave's LDS buffer (all of a
-// stage's reads into registers before its writes).
+// table e^{-2 pi i t / T} (T = max(n / 2, L)) that serves every FFT size.
+// The FFTs are Stockham autosort transforms (N. Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC 2008)
+// in radix 16 (a radix-2 and a radix-4 stage first where log2 N needs them),
+// in place in the wave's LDS buffer (all of a stage's reads into registers
+// before its writes).
 #include <algorithm>
 
 #include "common.h"
