@@ -843,6 +843,7 @@ int pycbf_das_synthetic_bwd(const pycbf_rfinfo *info, const float *rf, int64_t n
     b.rxpack = rxpack;
     b.pvec = pvec;
     b.c0 = c0;
+    b.rc0 = 1.0f / c0;
     DasArgs tb{};
     tb.geo = info->ndim;
     tb.txpack = txpack;

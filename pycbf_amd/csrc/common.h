@@ -165,7 +165,7 @@ struct BwdArgs {
     int geo;                       // 0: the tables above; 2 / 3: synthetic geometry of that
                                    // ndim, evaluated in registers (tables unused)
     const float *txpack, *rxpack, *pvec;  // synthetic geometry (geo > 0), pvec (np, geo)
-    float c0;
+    float c0, rc0;  // rc0 = 1 / c0 rounded
     double *gc0part;  // per-workgroup partial sums of d loss / d c0 (geo > 0), or null
     int gc0_off;      // first partial of this launch
 };

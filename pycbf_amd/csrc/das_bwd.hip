@@ -48,11 +48,16 @@ namespace {
 constexpr int BWD_BINS = 2048;  // LDS bins per array and channel window
 
 // value and derivative of the interpolant at x (0 outside the trace)
-template <int GK>
+// FD: the knot quotient by dt through div_rn (dt accepted by
+// recip_division_exact, rdt = 1 / dt rounded): the same index
+template <int GK, bool FD = false>
 __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t row, int n,
                                           float x, float t0, float dt, float rdt, float tmax,
                                           float &v, float &d)
 {
+    auto knot = [&](int &j) {
+        return FD ? knot_index_rn(x, t0, dt, rdt, tmax, n, j) : knot_index(x, t0, dt, tmax, n, j);
+    };
     if (GK == GK_NEAREST) {
         int it = 0;
         if (!nearest_index(x, t0, dt, n, it)) return false;
@@ -61,7 +66,7 @@ __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t
         return true;
     } else if (GK == GK_KORDER || GK == GK_AKIMA) {
         int j = 0;
-        if (!knot_index(x, t0, dt, tmax, n, j)) return false;
+        if (!knot(j)) return false;
         // the forward's roundings of t (korder_cubic / akima)
         const float t = (GK == GK_KORDER) ? ((x - t0) - dt * (float)j) * rdt
                                           : (x - (t0 + dt * (float)j)) * rdt;
@@ -71,7 +76,7 @@ __device__ __forceinline__ bool interp_vd(const float *__restrict__ src, int64_t
         return true;
     } else {
         int j = 0;
-        if (!knot_index(x, t0, dt, tmax, n, j)) return false;
+        if (!knot(j)) return false;
         const float xlead = t0 + dt * (float)(j + 1);
         const float xlag = t0 + dt * (float)j;
         if (GK == GK_LINEAR) {
@@ -153,7 +158,15 @@ __device__ __forceinline__ void bwd_pixel(const BwdArgs &A, int64_t p, float (&p
 // delay and weight of element e of the transmit (tx) or receive axis at pixel
 // p: the tables (GEO 0), or the synthetic geometry evaluated in registers with
 // the forward's float32 operations (GEO = ndim)
-template <int GEO>
+// knot index of the adjoint's trace (FD: the quotient by dt through div_rn)
+template <bool FD>
+__device__ __forceinline__ bool knot_at(const BwdArgs &A, float x, int &j)
+{
+    return FD ? knot_index_rn(x, A.t0, A.dt, A.rdt, A.tmax, A.nt, j)
+              : knot_index(x, A.t0, A.dt, A.tmax, A.nt, j);
+}
+
+template <int GEO, bool FD = false>
 __device__ __forceinline__ void bwd_elem(const BwdArgs &A, bool tx, int e, int64_t p,
                                          const float (&pv)[3], float &tau, float &apod)
 {
@@ -162,9 +175,9 @@ __device__ __forceinline__ void bwd_elem(const BwdArgs &A, bool tx, int e, int64
         tau = tx ? A.ttx[o] : A.trx[o];
         apod = tx ? A.atx[o] : A.arx[o];
     } else if (tx) {
-        geom_tx<GEO>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod);
+        geom_tx<GEO, FD>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
     } else {
-        geom_rx<GEO>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod);
+        geom_rx<GEO, FD>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
     }
 }
 
@@ -175,7 +188,7 @@ __device__ __forceinline__ void bwd_elem(const BwdArgs &A, bool tx, int e, int64
 // path length over c0, the transmit's time intercept included (pycbf's
 // helpers compute it as a distance over the sound speed,
 // pycbf/helpers/__full_aperture_2D__.py:259,300,346).
-template <int GK, int OUTER_TX, int GEO>
+template <int GK, int OUTER_TX, int GEO, bool FD>
 __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
 {
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -185,7 +198,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
     float pv[3];
     bwd_pixel<GEO>(A, act ? p : 0, pv);
     float tau_o = 0.0f, ap_o = 0.0f;
-    if (act) bwd_elem<GEO>(A, OUTER_TX, o, p, pv, tau_o, ap_o);
+    if (act) bwd_elem<GEO, FD>(A, OUTER_TX, o, p, pv, tau_o, ap_o);
     // delay residuals: the derivative is taken at the position in double
     const float *lo_o_t = OUTER_TX ? A.ttx_lo : A.trx_lo;
     const float *lo_i_t = OUTER_TX ? A.trx_lo : A.ttx_lo;
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
         if (GEO == 0) {
             ap_i = (OUTER_TX ? A.arx : A.atx)[(int64_t)i * A.np + p];
         } else {
-            bwd_elem<GEO>(A, !OUTER_TX, i, p, pv, tau_i, ap_i);
+            bwd_elem<GEO, FD>(A, !OUTER_TX, i, p, pv, tau_i, ap_i);
         }
         const float w = OUTER_TX ? ap_o * ap_i : ap_i * ap_o;
         if (!(fabsf(w) >= A.thresh)) continue;
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
         for (int f = 0; f < A.nfr; ++f) {
             const int64_t row = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
             float v = 0.0f, d = 0.0f;
-            if (!interp_vd<GK>(A.src, row, A.nt, x, A.t0, A.dt, A.rdt, A.tmax, v, d)) break;
+            if (!interp_vd<GK, FD>(A.src, row, A.nt, x, A.t0, A.dt, A.rdt, A.tmax, v, d)) break;
             if (fine) d = interp_d_at<GK>(A.src, row, A.nt, q, A.rdt);
             const float g = A.gout[gout_index(A.sumflag, f, tx, rx, A.ntx, A.nrx, A.np, p)];
             gt = __builtin_fmaf(g * w, d, gt);
@@ -318,7 +331,7 @@ __device__ __forceinline__ double bwd_scale(const unsigned *mbits, int per_bin =
 // LDS bins are 64-bit fixed-point integers (ds_add_u64): on gfx950 integer
 // LDS atomics run at about 37x the rate of float LDS atomics, and integer
 // sums do not depend on the order of the additions.
-template <int GK, int GEO>
+template <int GK, int GEO, bool FD>
 __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
 {
     __shared__ unsigned long long by[BWD_BINS];
@@ -335,7 +348,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
     float pv[3];
     bwd_pixel<GEO>(A, act ? p : 0, pv);
     float ttx = 0.0f, atx = 0.0f;
-    if (act) bwd_elem<GEO>(A, true, tx, p, pv, ttx, atx);
+    if (act) bwd_elem<GEO, FD>(A, true, tx, p, pv, ttx, atx);
     const double scale = bwd_scale(A.mbits);
     const bool fix = scale > 0.0;  // uniform
     const double iscale = fix ? 1.0 / scale : 0.0;
@@ -389,7 +402,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
         if (act) {
             float trx = 0.0f, arx = 0.0f;
             if (GEO == 0) arx = A.arx[(int64_t)rx * A.np + p];
-            else bwd_elem<GEO>(A, false, rx, p, pv, trx, arx);
+            else bwd_elem<GEO, FD>(A, false, rx, p, pv, trx, arx);
             const float w = atx * arx;
             if (fabsf(w) >= A.thresh) {
                 if (GEO == 0) trx = A.trx[(int64_t)rx * A.np + p];
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                     }
                 } else if (GK == GK_KORDER || GK == GK_AKIMA) {
                     int j = 0;
-                    if (knot_index(x, A.t0, A.dt, A.tmax, A.nt, j) && g != 0.0f) {
+                    if (knot_at<FD>(A, x, j) && g != 0.0f) {
                         const float t = (GK == GK_KORDER)
                                             ? ((x - A.t0) - A.dt * (float)j) * A.rdt
                                             : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
@@ -415,7 +428,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
                     }
                 } else {
                     int j = 0;
-                    if (knot_index(x, A.t0, A.dt, A.tmax, A.nt, j) && g != 0.0f) {
+                    if (knot_at<FD>(A, x, j) && g != 0.0f) {
                         const float xlead = A.t0 + A.dt * (float)(j + 1);
                         const float xlag = A.t0 + A.dt * (float)j;
                         float a, b;
@@ -475,7 +488,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_rf(BwdArgs A)
 // the order of the pixels.  16 waves per workgroup sweep 1024 pixels per
 // step; a wave skips its 64 pixels when the tile's delay range leaves the
 // channel unused.
-template <int GK, int GEO>
+template <int GK, int GEO, bool FD>
 __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
 {
     extern __shared__ unsigned long long bins[];
@@ -527,8 +540,8 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
         } else {
             float pv[3];
             bwd_pixel<GEO>(A, p, pv);
-            bwd_elem<GEO>(A, true, tx, p, pv, ttx, atx);
-            bwd_elem<GEO>(A, false, rx, p, pv, trx, arx);
+            bwd_elem<GEO, FD>(A, true, tx, p, pv, ttx, atx);
+            bwd_elem<GEO, FD>(A, false, rx, p, pv, trx, arx);
         }
         const float w = atx * arx;
         if (!(fabsf(w) >= A.thresh)) continue;
@@ -544,7 +557,7 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
             if (nearest_index(x, A.t0, A.dt, A.nt, it)) bin_add(by, byf, it, g);
         } else if (HERM) {
             int j = 0;
-            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
+            if (!knot_at<FD>(A, x, j)) continue;
             // |t| <= 1 (to rounding): every contribution stays within the bound
             const float t = (GK == GK_KORDER) ? ((x - A.t0) - A.dt * (float)j) * A.rdt
                                               : (x - (A.t0 + A.dt * (float)j)) * A.rdt;
@@ -554,7 +567,7 @@ __global__ __launch_bounds__(1024) void k_das_bwd_rf_ch(BwdArgs A)
             bin_add(b3, b3f, j, g * ((t * t) * t));
         } else {
             int j = 0;
-            if (!knot_index(x, A.t0, A.dt, A.tmax, A.nt, j)) continue;
+            if (!knot_at<FD>(A, x, j)) continue;
             const float xlead = A.t0 + A.dt * (float)(j + 1);
             const float xlag = A.t0 + A.dt * (float)j;
             float a, b;
@@ -1089,7 +1102,7 @@ static int fill_zero(float *p, int64_t n, hipStream_t s)
     return check_launch("fill zero");
 }
 
-template <int GK, int GEO>
+template <int GK, int GEO, bool FD>
 static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
                         float *gapodrx, float *grf, float *gc0, hipStream_t s)
 {
@@ -1102,7 +1115,7 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         b.gc0part = want_c0 ? a.gc0part : nullptr;
         b.gc0_off = 0;
         ProfScope ps("das_bwd_tau", s);
-        hipLaunchKernelGGL((k_das_bwd_tau<GK, 1, GEO>), dim3(ptiles, a.ntx), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 1, GEO, FD>), dim3(ptiles, a.ntx), dim3(256), 0, s, b);
         if (int rc = check_launch("das_bwd_tau")) return rc;
     }
     if (gtaurx || gapodrx || want_c0) {
@@ -1112,7 +1125,7 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         b.gc0part = want_c0 ? a.gc0part : nullptr;
         b.gc0_off = (int)(ptiles * (unsigned)a.ntx);
         ProfScope ps("das_bwd_tau", s);
-        hipLaunchKernelGGL((k_das_bwd_tau<GK, 0, GEO>), dim3(ptiles, a.nrx), dim3(256), 0, s, b);
+        hipLaunchKernelGGL((k_das_bwd_tau<GK, 0, GEO, FD>), dim3(ptiles, a.nrx), dim3(256), 0, s, b);
         if (int rc = check_launch("das_bwd_tau")) return rc;
     }
     if (want_c0) {
@@ -1145,7 +1158,7 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
         hipLaunchKernelGGL(k_bwd_scale, dim3(sb, 3), dim3(256), 0, s, a, ngout);
         if (int rc = check_launch("das_bwd_scale")) return rc;
         if (per_channel) {
-            hipLaunchKernelGGL((k_das_bwd_rf_ch<GK, GEO>), dim3((unsigned)nch), dim3(1024), ch_lds, s,
+            hipLaunchKernelGGL((k_das_bwd_rf_ch<GK, GEO, FD>), dim3((unsigned)nch), dim3(1024), ch_lds, s,
                                a);
         } else {
             if (GK == GK_CUBIC || GK == GK_KORDER || GK == GK_AKIMA) {
@@ -1160,7 +1173,7 @@ static int launch_bwd_t(const BwdArgs &a, float *gtautx, float *gapodtx, float *
                 // the cotangent rows (grf itself, or the upsampled grid's rows)
                 if (int rc = fill_zero(a.gy, nch * a.gstride, s)) return rc;
             }
-            hipLaunchKernelGGL((k_das_bwd_rf<GK, GEO>), dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_das_bwd_rf<GK, GEO, FD>), dim3(ptiles, a.ntx, a.nfr), dim3(256), 0, s,
                                a);
         }
         if (int rc = check_launch("das_bwd_rf")) return rc;
@@ -1264,22 +1277,35 @@ int launch_upsample_adjoint(const float *gu, int64_t nch, int32_t nt, int32_t us
     return check_launch("spline_adjoint");
 }
 
+template <int GEO, bool FD>
+static int launch_das_bwd_fd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx,
+                             float *gtaurx, float *gapodrx, float *grf, float *gc0, hipStream_t s)
+{
+    switch (gk) {
+    case GK_NEAREST:
+        return launch_bwd_t<GK_NEAREST, GEO, FD>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_LINEAR:
+        return launch_bwd_t<GK_LINEAR, GEO, FD>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_KORDER:
+        return launch_bwd_t<GK_KORDER, GEO, FD>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    case GK_AKIMA:
+        return launch_bwd_t<GK_AKIMA, GEO, FD>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    default:
+        return launch_bwd_t<GK_CUBIC, GEO, FD>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    }
+}
+
+// the quotients by dt (and by c0 with the geometry) as rounded-reciprocal
+// products where the host check accepts the divisors (the same indices and
+// delays as the divisions)
 template <int GEO>
 static int launch_das_bwd_geo(int gk, const BwdArgs &a, float *gtautx, float *gapodtx,
                               float *gtaurx, float *gapodrx, float *grf, float *gc0, hipStream_t s)
 {
-    switch (gk) {
-    case GK_NEAREST:
-        return launch_bwd_t<GK_NEAREST, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
-    case GK_LINEAR:
-        return launch_bwd_t<GK_LINEAR, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
-    case GK_KORDER:
-        return launch_bwd_t<GK_KORDER, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
-    case GK_AKIMA:
-        return launch_bwd_t<GK_AKIMA, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
-    default:
-        return launch_bwd_t<GK_CUBIC, GEO>(a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
-    }
+    const bool fd = a.rdt == 1.0f / a.dt && recip_division_exact(a.dt) &&
+                    (GEO == 0 || (a.rc0 == 1.0f / a.c0 && recip_division_exact(a.c0)));
+    if (fd) return launch_das_bwd_fd<GEO, true>(gk, a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
+    return launch_das_bwd_fd<GEO, false>(gk, a, gtautx, gapodtx, gtaurx, gapodrx, grf, gc0, s);
 }
 
 int launch_das_bwd(int gk, const BwdArgs &a, float *gtautx, float *gapodtx, float *gtaurx,
