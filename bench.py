@@ -206,6 +206,7 @@ def time_tabbed(bf, rf, ref_out, steps):
     import torch
 
     from pycbf_amd.gpu import TabbedBeamformer
+    from pycbf_amd.gpu import _engine as E
     tt, at, tr, ar = bf.tables()
     bft = TabbedBeamformer(tautx=tt, apodtx=at, taurx=tr, apodrx=ar, t0=0.0, dt=1.0 / 40e6,
                            nt=NT, interp={"kind": "cubic"}, sumtype="tx_and_rx",
@@ -214,15 +215,19 @@ def time_tabbed(bf, rf, ref_out, steps):
     outv = torch.empty_like(ref_out)
     bft(rf, out_as_numpy=False, buffer=outv)
     torch.cuda.synchronize()
+    E.lib.pycbf_profile_reset()
+    E.lib.pycbf_profile_enable(1)
     t0 = time.perf_counter()
     for _ in range(steps):
         bft(rf, out_as_numpy=False, buffer=outv)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    E.lib.pycbf_profile_enable(0)
+    gather_ms, _ = E.profile_read("das_gather")
     err = float((outv - ref_out).abs().max() / ref_out.abs().max())
     del bft, outv
     torch.cuda.empty_cache()
-    return ms, err
+    return ms, err, gather_ms
 
 
 SLSC_NXO, SLSC_NZO, SLSC_SUB, SLSC_LAGS, SLSC_KWIN = 256, 256, 8, list(range(1, 26)), 8
@@ -846,7 +851,7 @@ def main():
     lds_bytes = 16.0 * contrib * args.frames
     lds_achieved = lds_bytes / (gather_ms * 1e-3) / 1e9
     peaks = {"hbm": None, "l2": None, "lds": None}
-    plain_ms = plain_err = direct_ms = direct_err = tab_ms = tab_err = None
+    plain_ms = plain_err = direct_ms = direct_err = tab_ms = tab_err = tab_gather = None
     if not args.no_compare:
         peaks = {k: round(E.measure_bandwidth(k, dev), 1) for k in ("hbm", "l2", "lds")}
         log(f"measured read bandwidth GB/s: {peaks}; contributing samples per frame {contrib} "
@@ -854,8 +859,10 @@ def main():
         # plain-GPU comparators on the same workload and data
         plain_ms, plain_err = time_variant("PYCBF_DAS_PLAIN", "1", params, rf, out, 2)
         direct_ms, direct_err = time_variant("PYCBF_DAS_LDS", "0", params, rf, out, 2)
-        tab_ms, tab_err = time_tabbed(bf, rf, out, 3)
-        log(f"TabbedBeamformer on the same tables {tab_ms:.2f} ms/step (err {tab_err:.2e})")
+        tab_ms, tab_err, tab_gather = time_tabbed(bf, rf, out, 3)
+        tab_gather /= 3
+        log(f"TabbedBeamformer on the same tables {tab_ms:.2f} ms/step, gather "
+            f"{tab_gather:.3f} ms (err {tab_err:.2e})")
         log(f"thread-per-sample comparator {plain_ms:.2f} ms/step (err {plain_err:.2e}); direct "
             f"gather {direct_ms:.2f} ms/step (err {direct_err:.2e})")
 
@@ -940,7 +947,7 @@ def main():
                       "pycbf's GPU launch scheme (PYCBF_DAS_PLAIN=1)"},
         "tabbed_gpu": None if tab_ms is None else {
             "value": round(samples / (tab_ms * 1e-3) / 1e9, 2), "unit": "Gsamples/s",
-            "ms_per_step": round(tab_ms, 3),
+            "ms_per_step": round(tab_ms, 3), "gather_ms": round(tab_gather, 3),
             "max_err_rel_peak_vs_headline": float(f"{tab_err:.3g}"),
             "kernel": "TabbedBeamformer over the geometry's delay / apodisation tables "
                       "(table-derived pixel order)"},

@@ -25,6 +25,8 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "common.h"
 #include "das_dev.h"
 #include "prof.h"
@@ -400,19 +402,23 @@ constexpr int DMA_CHUNKS = 768;  // 16-byte chunks per buffer (12 KB), all frame
 typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
-template <int SUM, int FC, int GEO>
+template <int SUM, int FC, int GEO, bool OUTER_TX>
 __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
 {
     constexpr int CH = DMA_CHUNKS / FC;     // chunk slots per frame row: 48 (FC 16) or 192
     constexpr int WINMAX = 2 * CH - 1;      // entries a frame row can always hold
     constexpr int NI = DMA_CHUNKS / LDS_BLOCK;  // DMA instructions per wave and channel (3)
-    __shared__ __align__(16) float lds[2 * DMA_CHUNKS * 4];
+    // the two staging buffers are separate objects, so the compiler can tell a
+    // read of one from the DMA into the other and does not hold the read for
+    // that DMA's count
+    __shared__ __align__(16) float lds0[DMA_CHUNKS * 4];
+    __shared__ __align__(16) float lds1[DMA_CHUNKS * 4];
     __shared__ int2 wins[DMA_MAXB];
 
     uint32_t bx, by, bz;
     block_coords(A.xcd_remap, bx, by, bz);
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t i = (int64_t)bx * LDS_BLOCK + tid;
     const bool live = i < A.np;
     const int64_t p = live ? (A.perm ? (int64_t)A.perm[i] : i) : 0;
@@ -420,7 +426,8 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
     const int nf = min(FC, A.nfr - f0);
     const int64_t np = A.np;
 
-    const bool OUTER_TX = (SUM == PYCBF_SUM_RX_ONLY) || (SUM != PYCBF_SUM_TX_ONLY && A.outer_tx);
+    static_assert(SUM != PYCBF_SUM_RX_ONLY || OUTER_TX, "rx_only sums over the inner receives");
+    static_assert(SUM != PYCBF_SUM_TX_ONLY || !OUTER_TX, "tx_only sums over the inner transmits");
     const int na = OUTER_TX ? A.ntx : A.nrx;
     const int nb = OUTER_TX ? A.nrx : A.ntx;
     const int a0 = bz * A.a_split;
@@ -437,14 +444,51 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
 #pragma unroll
         for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
     }
-    // DMA role of this lane: chunk q = (k * 4 + wave) * 64 + lane of the block
-    int dma_f[NI], dma_c[NI];
+    // DMA role of this lane: chunk q = (k * 4 + wave) * 64 + lane of the block,
+    // slot dma_c of frame row q / CH, whose table offset (entries, frame part)
+    // is dma_fo; chunks of absent frames reload frame nf - 1
+    int dma_c[NI];
+    int64_t dma_fo[NI];
 #pragma unroll
     for (int k = 0; k < NI; ++k) {
         const int q = (k * (LDS_BLOCK / 64) + wave) * 64 + lane;
-        dma_f[k] = q / CH;
         dma_c[k] = q % CH;
+        dma_fo[k] = (int64_t)(f0 + min(q / CH, nf - 1)) * nchan * rowlen;
     }
+
+    // an inner element's delay and weight at this pixel.  Tables (GEO 0): the
+    // entries are per-pixel vector loads, fetched one item ahead (inner_fetch)
+    // so that no wait inside an item holds its evaluation behind the next
+    // item's DMA (both count in vmcnt).  Geometry: the element's pack comes in
+    // through the scalar cache (read-only for the kernel's lifetime), whose
+    // loads wait on lgkmcnt, not on the DMA.
+    constexpr int NIN = 2;
+    auto inner_fetch = [&](int b, float(&d)[NIN]) {
+        if (GEO == 0) {
+            const int64_t o = (int64_t)b * A.np + p;
+            d[0] = OUTER_TX ? A.trx[o] : A.ttx[o];
+            d[1] = OUTER_TX ? A.arx[o] : A.atx[o];
+        }
+    };
+    auto inner_eval = [&](int b, const float(&d)[NIN], float &tau, float &apod) {
+        typedef const __attribute__((address_space(4))) float cfloat;
+        if (GEO == 0) {
+            tau = d[0];
+            apod = d[1];
+        } else if (OUTER_TX) {
+            cfloat *cp = (cfloat *)(A.rxpack + (int64_t)b * PYCBF_RXPACK_FLOATS);
+            float pk[PYCBF_RXPACK_FLOATS];
+#pragma unroll
+            for (int i = 0; i < PYCBF_RXPACK_FLOATS; ++i) pk[i] = cp[i];
+            geom_rx<GEO, true>(pk, A.c0, pv, tau, apod, A.rc0);
+        } else {
+            cfloat *cp = (cfloat *)(A.txpack + (int64_t)b * PYCBF_TXPACK_FLOATS);
+            float pk[PYCBF_TXPACK_FLOATS];
+#pragma unroll
+            for (int i = 0; i < PYCBF_TXPACK_FLOATS; ++i) pk[i] = cp[i];
+            geom_tx<GEO, true>(pk, A.c0, pv, tau, apod, A.rc0);
+        }
+    };
 
     float acc[FC];
 #pragma unroll
@@ -473,45 +517,43 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             return w.y > 0 && w.y <= WINMAX;
         };
         // issue the DMA of channel b into buffer buf (workgroup-uniform condition)
-        auto dma = [&](int b, int buf) {
+        auto dma = [&](int b, auto bufc) {
+            constexpr int buf = decltype(bufc)::value;
             if (!staged(b)) return;
             const int2 w = wins[b];
+            // entry w.x of the channel's frame-0 row (workgroup-uniform)
+            const int64_t cho = row_of(b, -f0) * rowlen + w.x;
+            const float *base = A.src + cho * 2;
 #pragma unroll
             for (int k = 0; k < NI; ++k) {
-                const int f = min(dma_f[k], nf - 1);  // chunks of absent frames reload frame nf-1
-                const int64_t r = row_of(b, f);
-                const int par = (int)((r * rowlen + w.x) & 1);
-                const int s0 = w.x - par;  // 16-byte aligned first entry (row r-1's last if -1)
-                const int nch = (w.x + w.y - s0 + 1) >> 1;
-                const int c = min(dma_c[k], nch - 1);
-                const float *src = A.src + (r * rowlen + s0 + 2 * c) * 2;
-                float *dst = lds + (buf * DMA_CHUNKS + (k * (LDS_BLOCK / 64) + wave) * 64) * 4;
+                // the row's first staged entry is w.x - par, 16-byte aligned
+                // (row r-1's last if -1); its nch = (w.y + par + 1) / 2 chunks
+                const int par = (int)(((uint32_t)dma_fo[k] + (uint32_t)cho) & 1u);
+                const int c = min(dma_c[k], (w.y + par - 1) >> 1);
+                const float *src = base + (dma_fo[k] + (2 * c - par)) * 2;
+                float *dst = (buf ? lds1 : lds0) + (k * (LDS_BLOCK / 64) + wave) * 64 * 4;
                 __builtin_amdgcn_global_load_lds((glb_void *)src, (lds_void *)dst, 16, 0, 0);
             }
         };
-        auto compute = [&](int b, int buf) {
-            const int2 w = wins[b];
-            if (w.y == 0) return;
+        // delay tautx + taurx and weight apodtx * apodrx of channel b at this
+        // pixel from the inner element's inputs d
+        auto delay = [&](int b, const float(&d)[NIN], float &x, float &wgt) {
             float tb = 0.0f, wb = 0.0f;
-            if (live) elem_eval<GEO, true>(A, !OUTER_TX, b, p, pv, tb, wb);
-            const float wgt = OUTER_TX ? wa * wb : wb * wa;  // apodtx * apodrx
-            const float x = OUTER_TX ? ta + tb : tb + ta;    // tautx + taurx
-            if (!(live && fabsf(wgt) >= A.thresh)) return;
-            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
-            if (w.y > WINMAX) {
-                float v[FC];
-                bool ok = false;
-                gather<GK_CUBIC, FC>(A, (int64_t)f0 * nchan + (int64_t)tx * A.nrx + rx, fstride,
-                                     nf, x, ok, v);
-                if (ok) {
-#pragma unroll
-                    for (int f = 0; f < FC; ++f)
-                        if (f < nf) acc[f] = __builtin_fmaf(wgt, v[f], acc[f]);
-                }
-                return;
-            }
+            if (live) inner_eval(b, d, tb, wb);
+            wgt = OUTER_TX ? wa * wb : wb * wa;
+            x = OUTER_TX ? ta + tb : tb + ta;
+        };
+        // a staged channel (window of 1 .. WINMAX entries) from LDS buffer buf
+        auto compute = [&](int b, auto bufc, const float(&d)[NIN]) {
+            constexpr int buf = decltype(bufc)::value;
+            const int2 w = wins[b];
+            if (!(w.y > 0 && w.y <= WINMAX)) return;
+            float x, wgt;
+            delay(b, d, x, wgt);
             int j = 0;
-            if (!knot_index_rn(x, A.t0, A.dt, A.rdt, A.tmax, A.nsrc, j)) return;
+            if (!(live && fabsf(wgt) >= A.thresh &&
+                  knot_index_rn(x, A.t0, A.dt, A.rdt, A.tmax, A.nsrc, j)))
+                return;
             const float xlead = A.t0 + A.dt * (float)(j + 1);
             const float xlag = A.t0 + A.dt * (float)j;
             float ka = (xlead - x) * A.rdt;
@@ -529,7 +571,7 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             // every frame slot: those of frames past nf hold copies of frame
             // nf - 1 and feed sums that are never stored; straight-line code
             // lets the LDS reads of all frames overlap
-            const float *L = lds + buf * DMA_CHUNKS * 4 + (j - w.x) * 2;
+            const float *L = (buf ? lds1 : lds0) + (j - w.x) * 2;
 #pragma unroll
             for (int f = 0; f < FC; ++f) {
                 const int par = par0 ^ (fpar & f);
@@ -541,17 +583,50 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             }
         };
 
-        dma(0, 0);
-        __builtin_amdgcn_s_waitcnt(0);
-        if (staged(0)) __syncthreads();
-        for (int b = 0; b < nb; ++b) {
-            const int buf = b & 1;
-            dma(b + 1, buf ^ 1);
-            compute(b, buf);
+        bool wide = false;  // a channel's window exceeds the slot (workgroup-uniform)
+        // channel b: issue the DMA of channel b + 1 and the loads of its inner
+        // element's inputs (into dn, when some pixel of the tile takes a
+        // sample of it), evaluate channel b from LDS with inputs dc; the wait
+        // that ends the step covers both, so no wait inside a step holds its
+        // evaluation behind the next channel's DMA
+        auto step = [&](int b, auto bufc, const float(&dc)[NIN], float(&dn)[NIN]) {
+            constexpr int buf = decltype(bufc)::value;  // b & 1
+            dma(b + 1, std::integral_constant<int, buf ^ 1>());
+            if (b + 1 < nb && wins[b + 1].y > 0) inner_fetch(b + 1, dn);
+            wide |= wins[b].y > WINMAX;
+            compute(b, bufc, dc);
             // this wave's DMA of channel b + 1 has landed; the barrier makes
             // every wave's land, and frees buffer buf for channel b + 2
             __builtin_amdgcn_s_waitcnt(0);
             if (staged(b) || staged(b + 1)) __syncthreads();
+        };
+        float d0[NIN], d1[NIN];
+        if (wins[0].y > 0) inner_fetch(0, d0);
+        dma(0, std::integral_constant<int, 0>());
+        __builtin_amdgcn_s_waitcnt(0);
+        if (staged(0)) __syncthreads();
+        for (int b = 0; b < nb; b += 2) {
+            step(b, std::integral_constant<int, 0>(), d0, d1);
+            if (b + 1 < nb) step(b + 1, std::integral_constant<int, 1>(), d1, d0);
+        }
+        // channels whose window exceeds a staging slot: direct gathers, after
+        // the staged ones (a loop of its own keeps the staged loop's registers)
+        for (int b = 0; wide && b < nb; ++b) {
+            if (wins[b].y <= WINMAX) continue;
+            float x, wgt;
+            inner_fetch(b, d0);
+            delay(b, d0, x, wgt);
+            if (!(live && fabsf(wgt) >= A.thresh)) continue;
+            const int tx = OUTER_TX ? a : b, rx = OUTER_TX ? b : a;
+            float v[FC];
+            bool ok = false;
+            gather<GK_CUBIC, FC>(A, (int64_t)f0 * nchan + (int64_t)tx * A.nrx + rx, fstride, nf,
+                                 x, ok, v);
+            if (ok) {
+#pragma unroll
+                for (int f = 0; f < FC; ++f)
+                    if (f < nf) acc[f] = __builtin_fmaf(wgt, v[f], acc[f]);
+            }
         }
         if (SUM == PYCBF_SUM_TX_ONLY || SUM == PYCBF_SUM_RX_ONLY) {
 #pragma unroll
@@ -570,24 +645,29 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
     }
 }
 
-template <int SUM, int FC>
+template <int SUM, int FC, bool OT>
 static void cubic_dma_geo(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
 {
     if (geo == 3)
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 3>), grid, dim3(LDS_BLOCK), 0, s, a);
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 3, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
     else if (geo == 2)
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 2>), grid, dim3(LDS_BLOCK), 0, s, a);
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 2, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
     else
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 0>), grid, dim3(LDS_BLOCK), 0, s, a);
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 0, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
 }
 
 template <int FC>
 static void cubic_dma_sum(const DasArgs &a, dim3 grid, hipStream_t s)
 {
     switch (a.sumflag) {
-    case PYCBF_SUM_TX_ONLY: cubic_dma_geo<PYCBF_SUM_TX_ONLY, FC>(a.geo, a, grid, s); break;
-    case PYCBF_SUM_RX_ONLY: cubic_dma_geo<PYCBF_SUM_RX_ONLY, FC>(a.geo, a, grid, s); break;
-    default: cubic_dma_geo<PYCBF_SUM_TX_AND_RX, FC>(a.geo, a, grid, s); break;
+    case PYCBF_SUM_TX_ONLY: cubic_dma_geo<PYCBF_SUM_TX_ONLY, FC, false>(a.geo, a, grid, s); break;
+    case PYCBF_SUM_RX_ONLY: cubic_dma_geo<PYCBF_SUM_RX_ONLY, FC, true>(a.geo, a, grid, s); break;
+    default:
+        if (a.outer_tx)
+            cubic_dma_geo<PYCBF_SUM_TX_AND_RX, FC, true>(a.geo, a, grid, s);
+        else
+            cubic_dma_geo<PYCBF_SUM_TX_AND_RX, FC, false>(a.geo, a, grid, s);
+        break;
     }
 }
 
@@ -653,14 +733,6 @@ int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStr
     if (const char *e = getenv("PYCBF_XCD_REMAP")) a.xcd_remap = atoi(e);
     a.outer_tx = das_outer_is_tx(a.sumflag, a.ntx, a.nrx) ? 1 : 0;
     const int na = a.outer_tx ? a.ntx : a.nrx;
-    if (nsplit < 1) nsplit = 1;
-    if (nsplit > na) nsplit = na;
-    a.a_split = (na + nsplit - 1) / nsplit;
-    nsplit = (na + a.a_split - 1) / a.a_split;
-    const bool reduce = (a.sumflag == PYCBF_SUM_TX_AND_RX) && nsplit > 1;
-    float *final_out = a.out;
-    if (reduce) a.out = partial;
-    dim3 grid((unsigned)das_tiles(a.np), (unsigned)((a.nfr + fc - 1) / fc), (unsigned)nsplit);
     const int nb = a.outer_tx ? a.nrx : a.ntx;
     const bool dma = gk == GK_CUBIC && a.sumflag != PYCBF_SUM_NONE && (fc == 4 || fc == 16) &&
                      (a.sumflag == PYCBF_SUM_TX_AND_RX ? nb : (a.sumflag == PYCBF_SUM_TX_ONLY
@@ -669,6 +741,14 @@ int launch_das_lds(int gk, const DasArgs &a0, int nsplit, float *partial, hipStr
                      // its quotients by dt and c0 are products with the reciprocals
                      a.rdt == 1.0f / a.dt && recip_division_exact(a.dt) &&
                      (a.geo == 0 || (a.rc0 == 1.0f / a.c0 && recip_division_exact(a.c0)));
+    if (nsplit < 1) nsplit = 1;
+    if (nsplit > na) nsplit = na;
+    a.a_split = (na + nsplit - 1) / nsplit;
+    nsplit = (na + a.a_split - 1) / a.a_split;
+    const bool reduce = (a.sumflag == PYCBF_SUM_TX_AND_RX) && nsplit > 1;
+    float *final_out = a.out;
+    if (reduce) a.out = partial;
+    dim3 grid((unsigned)das_tiles(a.np), (unsigned)((a.nfr + fc - 1) / fc), (unsigned)nsplit);
     if (dma) {
         ProfScope prof("das_gather", s);
         if (fc == 16)

@@ -241,3 +241,58 @@ def test_headline_staged_register_gather(geo, rf20):
         ref = _oracle_frame(geo, rf20[f].cpu().numpy())
         assert_close_to_peak(got[f], ref, 1e-5, f"round-robin placement frame {f}")
 
+
+def _windows(tabs, perm, dt, es_pair):
+    """Sample window (table entries) of every (tile, tx, rx) of the processing
+    order, NaN where no pixel of the tile takes a sample of the channel."""
+    tautx, apodtx, taurx, apodrx = tabs
+    order = perm.reshape(-1, 256)
+
+    def rng(tau, apod):
+        t = np.where(apod != 0, tau, np.nan)[:, order]
+        return np.nanmin(t, axis=2), np.nanmax(t, axis=2)
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        txlo, txhi = rng(tautx, apodtx)
+        rxlo, rxhi = rng(taurx, apodrx)
+    lo = txlo[:, None, :] + rxlo[None, :, :]
+    hi = txhi[:, None, :] + rxhi[None, :, :]
+    jlo = np.floor(np.clip(lo, 0, None) / np.float32(dt))
+    jhi = np.floor(np.clip(hi, 0, None) / np.float32(dt))
+    return jhi - jlo + 1 + es_pair
+
+
+def test_dma_gather_wide_and_narrow_windows():
+    """Pixels in a fine patch (0.6 x 0.45 mm at 20 mm) and on a coarse grid over
+    4-12 mm, so some tiles' channel windows fit the 16-frame staging slot (95
+    entries) and others take the DMA kernel's direct-gather loop; 9 transmits
+    (an odd count: the last pass holds one), 10 frames, cubic tx_and_rx, every
+    pixel against the oracle."""
+    import torch
+    ntx, nrx, nt, nfr = 9, 32, 1024, 10
+    dt = 1.0 / sim.FS
+    xele = sim.linear_array(nrx)
+    steers = np.radians(np.linspace(-12, 12, ntx))
+    fx, fz = np.meshgrid(np.linspace(-0.3e-3, 0.3e-3, 32), np.linspace(20e-3, 20.45e-3, 16))
+    cx, cz = np.meshgrid(np.linspace(-5e-3, 5e-3, 32), np.linspace(4e-3, 12e-3, 16))
+    pnts = np.ascontiguousarray(np.stack([np.concatenate([fx.ravel(), cx.ravel()]),
+                                          np.concatenate([fz.ravel(), cz.ravel()])], axis=1))
+    params = sim.pw_synthetic(steers, xele, pnts)
+    p = params
+    tabs = oracle.synthetic_tables(p["ovectx"], p["nvectx"], p["t0tx"], p["alatx"], p["doftx"],
+                                   p["ovecrx"], p["nvecrx"], p["alarx"], p["c0"], pnts)
+    bf = SyntheticBeamformer(**params, t0=0.0, dt=dt, nt=nt, interp={"kind": "cubic"},
+                             sumtype="tx_and_rx", device="cuda:0")
+    win = _windows(tabs, bf._perm.cpu().numpy(), dt, 1)
+    print(f"wide/narrow windows: {int(np.sum(win > 95))} wide, {int(np.sum(win <= 95))} staged",
+          flush=True)
+    assert np.sum(win > 95) > 0 and np.sum(win <= 95) > 0
+    rng = np.random.default_rng(17)
+    rf = rng.standard_normal((nfr, ntx, nrx, nt)).astype(np.float32)
+    out = bf(torch.as_tensor(rf, device="cuda"), out_as_numpy=False)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for f in (0, 4, 9):
+        ref, _ = oracle.das_tabbed_mt(rf[f], 0.0, dt, *tabs, kind="cubic")
+        assert_close_to_peak(got[f], ref, 1e-5, f"wide/narrow frame {f}")
