@@ -402,7 +402,9 @@ constexpr int DMA_CHUNKS = 768;  // 16-byte chunks per buffer (12 KB), all frame
 typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
-template <int SUM, int FC, int GEO, bool OUTER_TX>
+// FODD: a frame's rows span an odd number of table entries, so the rows of
+// consecutive frames start on alternating 8-byte parities
+template <int SUM, int FC, int GEO, bool OUTER_TX, bool FODD>
 __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
 {
     constexpr int CH = DMA_CHUNKS / FC;     // chunk slots per frame row: 48 (FC 16) or 192
@@ -439,21 +441,25 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
     const int boff_a = OUTER_TX ? 0 : A.ntx;
     const int boff_b = OUTER_TX ? A.ntx : 0;
 
+    // pixel position (every lane: p is 0 past the end), and a full wait, so no
+    // load of the prologue is still counted when the channel loop's DMAs are
     float pv[3] = {0.0f, 0.0f, 0.0f};
-    if (GEO > 0 && live) {
 #pragma unroll
-        for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
-    }
+    for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+    __builtin_amdgcn_s_waitcnt(0);
     // DMA role of this lane: chunk q = (k * 4 + wave) * 64 + lane of the block,
-    // slot dma_c of frame row q / CH, whose table offset (entries, frame part)
-    // is dma_fo; chunks of absent frames reload frame nf - 1
-    int dma_c[NI];
-    int64_t dma_fo[NI];
+    // slot dma_c of frame row q / CH; dma_src is that frame's table (channel
+    // 0, entry 0) and dma_par its rows' parity relative to frame 0's (FODD);
+    // chunks of absent frames reload frame nf - 1
+    int dma_c[NI], dma_par[NI];
+    const float *dma_src[NI];
 #pragma unroll
     for (int k = 0; k < NI; ++k) {
         const int q = (k * (LDS_BLOCK / 64) + wave) * 64 + lane;
+        const int f = min(q / CH, nf - 1);
         dma_c[k] = q % CH;
-        dma_fo[k] = (int64_t)(f0 + min(q / CH, nf - 1)) * nchan * rowlen;
+        dma_par[k] = FODD ? (f & 1) : 0;
+        dma_src[k] = A.src + (int64_t)(f0 + f) * nchan * rowlen * 2;
     }
 
     // an inner element's delay and weight at this pixel.  Tables (GEO 0): the
@@ -521,16 +527,17 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             constexpr int buf = decltype(bufc)::value;
             if (!staged(b)) return;
             const int2 w = wins[b];
-            // entry w.x of the channel's frame-0 row (workgroup-uniform)
+            // entry w.x of the channel's row, offset from the frame's table and
+            // its parity in frame 0 (workgroup-uniform)
             const int64_t cho = row_of(b, -f0) * rowlen + w.x;
-            const float *base = A.src + cho * 2;
+            const int cpar = (int)(cho & 1);
 #pragma unroll
             for (int k = 0; k < NI; ++k) {
                 // the row's first staged entry is w.x - par, 16-byte aligned
                 // (row r-1's last if -1); its nch = (w.y + par + 1) / 2 chunks
-                const int par = (int)(((uint32_t)dma_fo[k] + (uint32_t)cho) & 1u);
+                const int par = FODD ? (dma_par[k] ^ cpar) : cpar;
                 const int c = min(dma_c[k], (w.y + par - 1) >> 1);
-                const float *src = base + (dma_fo[k] + (2 * c - par)) * 2;
+                const float *src = dma_src[k] + (cho - par) * 2 + 4 * c;
                 float *dst = (buf ? lds1 : lds0) + (k * (LDS_BLOCK / 64) + wave) * 64 * 4;
                 __builtin_amdgcn_global_load_lds((glb_void *)src, (lds_void *)dst, 16, 0, 0);
             }
@@ -567,14 +574,13 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             // row parity of frame f: that of frame 0, flipped every frame when a
             // frame's rows span an odd number of entries
             const int par0 = (int)((row_of(b, 0) * rowlen + w.x) & 1);
-            const int fpar = (int)((nchan * rowlen) & 1);
             // every frame slot: those of frames past nf hold copies of frame
             // nf - 1 and feed sums that are never stored; straight-line code
             // lets the LDS reads of all frames overlap
             const float *L = (buf ? lds1 : lds0) + (j - w.x) * 2;
 #pragma unroll
             for (int f = 0; f < FC; ++f) {
-                const int par = par0 ^ (fpar & f);
+                const int par = FODD ? par0 ^ (f & 1) : par0;
                 const float *Ef = L + f * CH * 4 + par * 2;
                 float r = __builtin_fmaf(ka, Ef[0], acc[f]);
                 r = __builtin_fmaf(kb, Ef[2], r);
@@ -645,15 +651,25 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
     }
 }
 
+template <int SUM, int FC, bool OT, bool FODD>
+static void cubic_dma_launch(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
+{
+    if (geo == 3)
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 3, OT, FODD>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else if (geo == 2)
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 2, OT, FODD>), grid, dim3(LDS_BLOCK), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 0, OT, FODD>), grid, dim3(LDS_BLOCK), 0, s, a);
+}
+
 template <int SUM, int FC, bool OT>
 static void cubic_dma_geo(int geo, const DasArgs &a, dim3 grid, hipStream_t s)
 {
-    if (geo == 3)
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 3, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
-    else if (geo == 2)
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 2, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
+    const int64_t frame_entries = (int64_t)a.ntx * a.nrx * (a.ch_stride / 2);
+    if (frame_entries & 1)
+        cubic_dma_launch<SUM, FC, OT, true>(geo, a, grid, s);
     else
-        hipLaunchKernelGGL((k_das_cubic_dma<SUM, FC, 0, OT>), grid, dim3(LDS_BLOCK), 0, s, a);
+        cubic_dma_launch<SUM, FC, OT, false>(geo, a, grid, s);
 }
 
 template <int FC>

@@ -266,11 +266,12 @@ def _windows(tabs, perm, dt, es_pair):
 def test_dma_gather_wide_and_narrow_windows():
     """Pixels in a fine patch (0.6 x 0.45 mm at 20 mm) and on a coarse grid over
     4-12 mm, so some tiles' channel windows fit the 16-frame staging slot (95
-    entries) and others take the DMA kernel's direct-gather loop; 9 transmits
-    (an odd count: the last pass holds one), 10 frames, cubic tx_and_rx, every
-    pixel against the oracle."""
+    entries) and others take the DMA kernel's direct-gather loop; 9 transmits x
+    31 receives x 1025 table entries (an odd count per frame: consecutive
+    frames' rows start on alternating parities), 10 frames, cubic tx_and_rx,
+    every pixel against the oracle."""
     import torch
-    ntx, nrx, nt, nfr = 9, 32, 1024, 10
+    ntx, nrx, nt, nfr = 9, 31, 1024, 10
     dt = 1.0 / sim.FS
     xele = sim.linear_array(nrx)
     steers = np.radians(np.linspace(-12, 12, ntx))
