@@ -354,8 +354,8 @@ int pycbf_ray_delays(const float *slow, int nzm, int nxm, float xm0, float zm0, 
                      float *tautx, float *taurx, float *tautx_lo, float *taurx_lo,
                      void *stream);
 
-/* Workspace bytes of pycbf_ray_delays_bwd (64-bit partial maps of its
- * workgroups). */
+/* Workspace bytes of pycbf_ray_delays_bwd (256 bytes, then one 64-bit
+ * fixed-point sum per map node). */
 size_t pycbf_ray_delays_bwd_workspace_bytes(int nzm, int nxm);
 
 /* Adjoint of pycbf_ray_delays: gslow (nzm, nxm) = d loss / d slow given

@@ -883,6 +883,8 @@ static int ray_args(const float *slow, int nzm, int nxm, float xm0, float zm0, f
     a.zm0d = (double)zm0;
     a.dxmd = (double)dxm;
     a.dzmd = (double)dzm;
+    a.rdxmd = 1.0 / (double)dxm;
+    a.rdzmd = 1.0 / (double)dzm;
     a.steer = steer;
     a.xorig = xorig;
     a.xele = xele;

@@ -212,12 +212,13 @@ struct RayArgs {
     const float *pvec;  // (np, 2)
     int64_t np;
     double xm0d, zm0d, dxmd, dzmd;     // map geometry in double (forward)
+    double rdxmd, rdzmd;               // 1 / dxm, 1 / dzm in double
     float *tautx, *taurx;              // forward outputs
     float *tautx_lo, *taurx_lo;        // forward residuals (may be null)
     const float *gtautx, *gtaurx;      // adjoint inputs
     float *gslow;                      // adjoint output
-    float *gpart;                      // adjoint scratch: 256 B (max word), then the workgroups'
-                                       // 64-bit fixed-point partial maps
+    float *gpart;                      // adjoint scratch: 256 B (max word), then the 64-bit
+                                       // fixed-point map sums
 };
 
 int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s);

@@ -29,7 +29,7 @@ namespace pycbf {
 namespace {
 
 constexpr int SOS_MAXMAP = 8192;    // map nodes held in LDS (64 KB of 64-bit sums in the adjoint)
-constexpr int SOS_PARTIALS = 1024;  // workgroups (partial maps) of the adjoint
+constexpr int SOS_PARTIALS = 1024;  // workgroups of the adjoint
 
 // bilinear sample positions: node indices and weights of point (x, z)
 __device__ __forceinline__ void bilin(const MapGeo &g, float x, float z, int &i0, int &k0,
@@ -66,13 +66,12 @@ __device__ __forceinline__ void ray_of(int e, int ntx, const float *steer, const
 
 }  // namespace
 
-// the same in double, from the float32 inputs and the map geometry in double
-__device__ __forceinline__ void bilin_d(const RayArgs &a, double x, double z, int &i0, int &k0,
+// the same in double, from the map coordinates u = (x - xm0) / dxm - 1/2 and
+// v = (z - zm0) / dzm - 1/2 of the point
+__device__ __forceinline__ void bilin_d(const RayArgs &a, double u, double v, int &i0, int &k0,
                                         double &wz, double &wx)
 {
     const int nxm = a.g.nxm, nzm = a.g.nzm;
-    double u = (x - a.xm0d) / a.dxmd - 0.5;
-    double v = (z - a.zm0d) / a.dzmd - 0.5;
     u = fmin(fmax(u, 0.0), (double)(nxm - 1));
     v = fmin(fmax(v, 0.0), (double)(nzm - 1));
     k0 = min((int)u, max(nxm - 2, 0));
@@ -98,9 +97,12 @@ __device__ __forceinline__ void ray_of_d(const RayArgs &a, int e, double px, dou
     }
 }
 
-__global__ void k_fill_u32(unsigned *p, int n, unsigned v)
+// clears the adjoint's max word and its n 64-bit map sums
+__global__ void k_ray_clear(unsigned *mbits, unsigned long long *gsum, int n)
 {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *mbits = 0u;
+    for (int i = t; i < n; i += gridDim.x * blockDim.x) gsum[i] = 0ull;
 }
 
 // one lane per (element, pixel); the map is staged in LDS
@@ -117,20 +119,23 @@ __global__ __launch_bounds__(256) void k_ray_delays(RayArgs a)
     double qx, L, fire;
     ray_of_d(a, e, px, pz, qx, L, fire);
     const int nxm = a.g.nxm, nzm = a.g.nzm;
+    // map coordinates along the ray, affine in t: u = u0 + t du, v = v0 + t dv
+    const double rn = 1.0 / (double)a.nsteps;
+    const double u0 = (qx - a.xm0d) * a.rdxmd - 0.5, du = (px - qx) * a.rdxmd;
+    const double v0 = -a.zm0d * a.rdzmd - 0.5, dv = pz * a.rdzmd;
     double acc = 0.0;
     for (int m = 0; m < a.nsteps; ++m) {
-        const double t = ((double)m + 0.5) / (double)a.nsteps;
-        const double x = qx + t * (px - qx), z = t * pz;
+        const double t = ((double)m + 0.5) * rn;
         int i0, k0;
         double wz, wx;
-        bilin_d(a, x, z, i0, k0, wz, wx);
+        bilin_d(a, fma(t, du, u0), fma(t, dv, v0), i0, k0, wz, wx);
         const int i1 = min(i0 + 1, nzm - 1), k1 = min(k0 + 1, nxm - 1);
         const double s00 = sm[i0 * nxm + k0], s01 = sm[i0 * nxm + k1];
         const double s10 = sm[i1 * nxm + k0], s11 = sm[i1 * nxm + k1];
         const double top = s00 + wx * (s01 - s00), bot = s10 + wx * (s11 - s10);
         acc += top + wz * (bot - top);
     }
-    const double tau = fire + L * (acc / (double)a.nsteps);
+    const double tau = fire + L * (acc * rn);
     const float hi = (float)tau;
     const float lo = (float)(tau - (double)hi);
     if (e < a.ntx) {
@@ -148,10 +153,10 @@ __global__ __launch_bounds__(256) void k_ray_delays(RayArgs a)
 // k_ray_scale finds M = max over rays of |g| * L, which bounds what one ray
 // adds to any node (its bilinear weights sum to at most nsteps per node and
 // gl = g L / nsteps), so |sum over all rays| <= total * M, and
-// scale = 2^62 / (total * M) keeps every partial and final sum within int64.
-// The rounding error per contribution is at most 2^-63 * total * M, and
-// integer sums do not depend on the order of the atomics: the map gradient
-// is deterministic.
+// scale = 2^62 / (max(total, 2^12) * M) keeps every partial and final sum
+// within int64.  The rounding error per contribution is at most
+// 2^-63 * max(total, 2^12) * M, and integer sums do not depend on the order
+// of the atomics: the map gradient is deterministic.
 __global__ __launch_bounds__(256) void k_ray_scale(RayArgs a, unsigned *mbits)
 {
     const int64_t total = (int64_t)(a.ntx + a.nrx) * a.np;
@@ -176,11 +181,21 @@ __global__ __launch_bounds__(256) void k_ray_scale(RayArgs a, unsigned *mbits)
     if ((threadIdx.x & 63) == 0) atomicMax(mbits, __float_as_uint(m));
 }
 
+// scale = 2^62 / (max(total, 2^12) * M): every sum fits int64, and one ray's
+// contribution to a node stays below 2^51, where fixed_ll converts exactly
 __device__ __forceinline__ double ray_scale(const RayArgs &a, const unsigned *mbits)
 {
     const float m = __uint_as_float(*mbits);
-    const double total = (double)(a.ntx + a.nrx) * (double)a.np;
+    const double total = fmax((double)(a.ntx + a.nrx) * (double)a.np, 4096.0);
     return (m > 0.0f && m <= 3.0e38f) ? 4611686018427387904.0 / (total * (double)m) : 0.0;
+}
+
+// x rounded to the nearest integer (ties to even) for |x| < 2^51: adding
+// 1.5 * 2^52 leaves the integer in the low significand bits
+__device__ __forceinline__ long long fixed_ll(double x)
+{
+    const double magic = 6755399441055744.0;
+    return __double_as_longlong(x + magic) - __double_as_longlong(magic);
 }
 
 // adjoint: one lane per ray.  Rays are visited in a scrambled order
@@ -188,9 +203,9 @@ __device__ __forceinline__ double ray_scale(const RayArgs &a, const unsigned *mb
 // its ray at a different step, so that the lanes of a wave follow unrelated
 // rays at different depths; along its ray a lane accumulates the bilinear
 // weights of the current cell in registers and adds them to the LDS map only
-// when the ray leaves the cell.  Each workgroup stores its LDS map as a
-// partial map, and k_ray_reduce sums the partials (no device-wide atomics on
-// the few map nodes).
+// when the ray leaves the cell.  Each workgroup adds its LDS map into the
+// global 64-bit sums with integer atomics (exact, so the order does not
+// matter), and k_ray_reduce scales them.
 __device__ __forceinline__ void flush_cell(unsigned long long *gm, int nxm, int nzm, int i0, int k0,
                                            float gl, double scale, float a00, float a01,
                                            float a10, float a11)
@@ -200,7 +215,7 @@ __device__ __forceinline__ void flush_cell(unsigned long long *gm, int nxm, int 
     // weights that stayed zero over the run (clamped edges) add nothing
     auto add = [&](int idx, float w) {
         if (w != 0.0f)
-            atomicAdd(&gm[idx], (unsigned long long)__double2ll_rn((double)(gl * w) * scale));
+            atomicAdd(&gm[idx], (unsigned long long)fixed_ll((double)(gl * w) * scale));
     };
     add(i0 * nxm + k0, a00);
     add(i0 * nxm + k1, a01);
@@ -209,7 +224,7 @@ __device__ __forceinline__ void flush_cell(unsigned long long *gm, int nxm, int 
 }
 
 __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigned *mbits,
-                                                        long long *gpart, int64_t stride,
+                                                        unsigned long long *gsum, int64_t stride,
                                                         int64_t step)
 {
     extern __shared__ unsigned long long gm64[];
@@ -265,15 +280,17 @@ __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigne
         }
     }
     __syncthreads();
-    long long *part = gpart + (int64_t)blockIdx.x * nmap;
-    for (int t = threadIdx.x; t < nmap; t += 256) part[t] = (long long)gm64[t];
+    for (int t = threadIdx.x; t < nmap; t += 256) {
+        const unsigned long long v = gm64[t];
+        if (v) atomicAdd(gsum + t, v);
+    }
 }
 
-// gslow[t] = (sum over workgroups of their partial maps) / scale; integer
-// sums are exact, so the result does not depend on the workgroup count or
-// order.  A non-finite upstream gradient gives NaN everywhere.
+// gslow[t] = (integer map sum) / scale; integer sums are exact, so the
+// result does not depend on the workgroup count or order.  A non-finite
+// upstream gradient gives NaN everywhere.
 __global__ __launch_bounds__(256) void k_ray_reduce(RayArgs a, const unsigned *mbits,
-                                                    const long long *__restrict__ part, int nb,
+                                                    const unsigned long long *__restrict__ gsum,
                                                     int nmap, float *__restrict__ gslow)
 {
     const int t = blockIdx.x * 256 + threadIdx.x;
@@ -288,9 +305,7 @@ __global__ __launch_bounds__(256) void k_ray_reduce(RayArgs a, const unsigned *m
         gslow[t] = 0.0f;
         return;
     }
-    long long acc = 0;
-    for (int b = 0; b < nb; ++b) acc += part[(int64_t)b * nmap + t];
-    gslow[t] = (float)((double)acc / scale);
+    gslow[t] = (float)((double)(long long)gsum[t] / scale);
 }
 
 int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s)
@@ -305,11 +320,13 @@ int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s)
     ProfScope ps("ray_delays_bwd", s);
     const int64_t total = (int64_t)(a.ntx + a.nrx) * a.np;
     const int nb = (int)std::min<int64_t>((total + 255) / 256, SOS_PARTIALS);
-    // workspace: the max word, then the partial maps (64-bit)
+    // workspace: the max word, then the map sums (64-bit)
     unsigned *mbits = reinterpret_cast<unsigned *>(a.gpart);
-    long long *part = reinterpret_cast<long long *>(reinterpret_cast<char *>(a.gpart) + 256);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, mbits, 1, 0u);
-    if (int rc = check_launch("ray_scale clear")) return rc;
+    unsigned long long *gsum =
+        reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(a.gpart) + 256);
+    hipLaunchKernelGGL(k_ray_clear, dim3((unsigned)((nmap + 255) / 256)), dim3(256), 0, s, mbits,
+                       gsum, nmap);
+    if (int rc = check_launch("ray adjoint clear")) return rc;
     const unsigned sb = (unsigned)std::min<int64_t>((total + 255) / 256, 2048);
     hipLaunchKernelGGL(k_ray_scale, dim3(sb), dim3(256), 0, s, a, mbits);
     if (int rc = check_launch("ray_scale")) return rc;
@@ -318,14 +335,14 @@ int launch_ray_delays(const RayArgs &a, int adjoint, hipStream_t s)
     while (std::__gcd(stride, total) != 1) ++stride;
     const int64_t step = (int64_t)(((unsigned __int128)((int64_t)nb * 256) * stride) % total);
     hipLaunchKernelGGL(k_ray_delays_bwd, dim3((unsigned)nb), dim3(256), (size_t)nmap * 8, s, a,
-                       mbits, part, stride, step);
+                       mbits, gsum, stride, step);
     if (int rc = check_launch("ray_delays_bwd")) return rc;
     hipLaunchKernelGGL(k_ray_reduce, dim3((unsigned)((nmap + 255) / 256)), dim3(256), 0, s, a,
-                       mbits, part, nb, nmap, a.gslow);
+                       mbits, gsum, nmap, a.gslow);
     return check_launch("ray_reduce");
 }
 
 int sos_max_map() { return SOS_MAXMAP; }
-size_t ray_bwd_workspace_bytes(int nzm, int nxm) { return 256 + (size_t)SOS_PARTIALS * nzm * nxm * 8; }
+size_t ray_bwd_workspace_bytes(int nzm, int nxm) { return 256 + (size_t)nzm * nxm * 8; }
 
 }  // namespace pycbf
