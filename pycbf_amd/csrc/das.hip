@@ -108,14 +108,34 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
     }
 }
 
+// out[g] = part[0][g] + part[1][g] + ... in split order.  V4: four
+// consecutive outputs per lane with 16-byte accesses (n a multiple of 4, both
+// buffers 16-byte aligned); the unrolled loop keeps eight loads in flight.
+template <bool V4>
 __global__ __launch_bounds__(256) void k_reduce_split(const float *__restrict__ part, int nsplit,
                                                       int64_t n, float *__restrict__ out)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    float s = part[g];
-    for (int z = 1; z < nsplit; ++z) s += part[(int64_t)z * n + g];
-    out[g] = s;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (V4) {
+        const int64_t g = 4 * t;
+        if (g >= n) return;
+        float4 s = *reinterpret_cast<const float4 *>(part + g);
+#pragma unroll 8
+        for (int z = 1; z < nsplit; ++z) {
+            const float4 v = *reinterpret_cast<const float4 *>(part + (int64_t)z * n + g);
+            s.x += v.x;
+            s.y += v.y;
+            s.z += v.z;
+            s.w += v.w;
+        }
+        *reinterpret_cast<float4 *>(out + g) = s;
+    } else {
+        if (t >= n) return;
+        float s = part[t];
+#pragma unroll 8
+        for (int z = 1; z < nsplit; ++z) s += part[(int64_t)z * n + t];
+        out[t] = s;
+    }
 }
 
 template <int GK, int SUM, int FC>
@@ -198,8 +218,14 @@ int launch_reduce_split(const float *part, int nsplit, int64_t n, float *out, hi
 {
     if (n <= 0) return 0;
     ProfScope prof("reduce_split", s);
-    hipLaunchKernelGGL(k_reduce_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
-                       nsplit, n, out);
+    const bool v4 = (n & 3) == 0 && (reinterpret_cast<uintptr_t>(part) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    if (v4)
+        hipLaunchKernelGGL(k_reduce_split<true>, dim3((unsigned)((n / 4 + 255) / 256)), dim3(256),
+                           0, s, part, nsplit, n, out);
+    else
+        hipLaunchKernelGGL(k_reduce_split<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           s, part, nsplit, n, out);
     return check_launch("das split reduce");
 }
 
