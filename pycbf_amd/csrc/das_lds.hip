@@ -55,43 +55,55 @@ struct Stage {
 };
 
 // ---------------------------------------------------------------------------
-// delay range per (tile, element) over the pixels that can contribute
+// delay range per (tile, element) over the pixels that can contribute; a
+// workgroup takes one tile and TB_ELEMS elements (one barrier for all of them)
 // ---------------------------------------------------------------------------
+constexpr int TB_ELEMS = 8;
+
 template <int GEO>
 __global__ __launch_bounds__(256) void k_tile_bounds(DasArgs A)
 {
     const int64_t tile = blockIdx.x;
-    const int e = blockIdx.y;  // [0, ntx): tx element, [ntx, ntx+nrx): rx element
+    const int e0 = blockIdx.y * TB_ELEMS;
+    const int ne = A.ntx + A.nrx;
     const int64_t i = tile * 256 + threadIdx.x;
-    float lo = INFINITY, hi = -INFINITY;
-    if (i < A.np) {
-        const int64_t p = A.perm ? (int64_t)A.perm[i] : i;
-        float pv[3] = {0.0f, 0.0f, 0.0f};
+    const bool in = i < A.np;
+    const int64_t p = in ? (A.perm ? (int64_t)A.perm[i] : i) : 0;
+    float pv[3] = {0.0f, 0.0f, 0.0f};
+    if (in) {
 #pragma unroll
         for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
-        const bool tx = e < A.ntx;
-        float tau = 0.0f, apod = 0.0f;
-        elem_eval<GEO>(A, tx, tx ? e : e - A.ntx, p, pv, tau, apod);
-        // with thresh > 0 a zero (or NaN) weight on either axis never contributes
-        const bool used = (tau == tau) && (!(A.thresh > 0.0f) || (apod != 0.0f && apod == apod));
-        if (used) lo = hi = tau;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lo = fminf(lo, __shfl_xor(lo, off, 64));
-        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
-    }
-    __shared__ float wl[4], wh[4];
+    __shared__ float wl[TB_ELEMS][4], wh[TB_ELEMS][4];
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        wl[w] = lo;
-        wh[w] = hi;
+    for (int k = 0; k < TB_ELEMS && e0 + k < ne; ++k) {
+        const int e = e0 + k;  // [0, ntx): tx element, [ntx, ntx+nrx): rx element
+        float lo = INFINITY, hi = -INFINITY;
+        if (in) {
+            const bool tx = e < A.ntx;
+            float tau = 0.0f, apod = 0.0f;
+            elem_eval<GEO>(A, tx, tx ? e : e - A.ntx, p, pv, tau, apod);
+            // with thresh > 0 a zero (or NaN) weight on either axis never contributes
+            const bool used =
+                (tau == tau) && (!(A.thresh > 0.0f) || (apod != 0.0f && apod == apod));
+            if (used) lo = hi = tau;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, off, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wl[k][w] = lo;
+            wh[k][w] = hi;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        lo = fminf(fminf(wl[0], wl[1]), fminf(wl[2], wl[3]));
-        hi = fmaxf(fmaxf(wh[0], wh[1]), fmaxf(wh[2], wh[3]));
-        float *b = const_cast<float *>(A.bounds) + (tile * (A.ntx + A.nrx) + e) * 2;
+    const int k = threadIdx.x;
+    if (k < TB_ELEMS && e0 + k < ne) {
+        const float lo = fminf(fminf(wl[k][0], wl[k][1]), fminf(wl[k][2], wl[k][3]));
+        const float hi = fmaxf(fmaxf(wh[k][0], wh[k][1]), fmaxf(wh[k][2], wh[k][3]));
+        float *b = const_cast<float *>(A.bounds) + (tile * ne + e0 + k) * 2;
         b[0] = lo;
         b[1] = hi;
     }
@@ -101,7 +113,7 @@ int launch_tile_bounds(const DasArgs &a, float *bounds, hipStream_t s)
 {
     DasArgs b = a;
     b.bounds = bounds;
-    dim3 grid((unsigned)das_tiles(a.np), (unsigned)(a.ntx + a.nrx));
+    dim3 grid((unsigned)das_tiles(a.np), (unsigned)((a.ntx + a.nrx + TB_ELEMS - 1) / TB_ELEMS));
     ProfScope prof("tile_bounds", s);
     if (a.geo == 3)
         hipLaunchKernelGGL(k_tile_bounds<3>, grid, dim3(256), 0, s, b);
