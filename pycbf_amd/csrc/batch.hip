@@ -98,14 +98,18 @@ __device__ __forceinline__ void batch_item(const BatchReqDev &R, int tile, int r
             for (int k = 0; k < BATCH_TB; ++k) {
                 ttx[k] = 0.0f;
                 atx[k] = 0.0f;
-                if (t0 + k < R.ntx)
-                    geom_tx<2>(R.txpack + (int64_t)(t0 + k) * PYCBF_TXPACK_FLOATS, R.c0, pv,
-                               ttx[k], atx[k]);
+                if (t0 + k < R.ntx) {
+                    float pk[PYCBF_TXPACK_FLOATS];
+                    load_pack(R.txpack + (int64_t)(t0 + k) * PYCBF_TXPACK_FLOATS, pk);
+                    geom_tx<2>(pk, R.c0, pv, ttx[k], atx[k]);
+                }
             }
 #pragma unroll 1
             for (int i = 0; i < nrc; ++i) {
                 float trx = 0.0f, arx = 0.0f;
-                geom_rx<2>(R.rxpack + (int64_t)(r0 + i) * PYCBF_RXPACK_FLOATS, R.c0, pv, trx, arx);
+                float pk[PYCBF_RXPACK_FLOATS];
+                load_pack(R.rxpack + (int64_t)(r0 + i) * PYCBF_RXPACK_FLOATS, pk);
+                geom_rx<2>(pk, R.c0, pv, trx, arx);
                 const float *base = R.coef + ((int64_t)t0 * R.nrx + r0 + i) * rowlen;
                 float part = 0.0f;
 #pragma unroll

@@ -82,6 +82,19 @@ __device__ __forceinline__ void block_coords(int remap, uint32_t &bx, uint32_t &
 // synthetic-point geometry (pycbf/gpu/__engines__.cu:382-470), float32, unfused
 // ---------------------------------------------------------------------------
 
+// An element's geometry pack, read through the constant address space: the
+// packs are read-only while a kernel runs, so a pack whose index is uniform
+// comes in through the scalar cache (its loads wait on lgkmcnt, not behind
+// the vector-memory loads in flight).
+template <int N>
+__device__ __forceinline__ void load_pack(const float *p, float (&pk)[N])
+{
+    typedef const __attribute__((address_space(4))) float cfloat;
+    const cfloat *cp = (const cfloat *)p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) pk[i] = cp[i];
+}
+
 // "acos(v) > ala" for the T of the element's pack; a NaN acos (|v| > 1) is false.
 __device__ __forceinline__ bool acos_gt(float v, float T) { return (v >= -1.0f && v <= 1.0f) && (v < T); }
 // "acos(v) <= ala"
@@ -163,9 +176,13 @@ __device__ __forceinline__ void elem_eval(const DasArgs &A, bool tx, int e, int6
         tau = tx ? A.ttx[o] : A.trx[o];
         apod = tx ? A.atx[o] : A.arx[o];
     } else if (tx) {
-        geom_tx<GEO, FD>(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
+        float pk[PYCBF_TXPACK_FLOATS];
+        load_pack(A.txpack + (int64_t)e * PYCBF_TXPACK_FLOATS, pk);
+        geom_tx<GEO, FD>(pk, A.c0, pv, tau, apod, A.rc0);
     } else {
-        geom_rx<GEO, FD>(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, A.c0, pv, tau, apod, A.rc0);
+        float pk[PYCBF_RXPACK_FLOATS];
+        load_pack(A.rxpack + (int64_t)e * PYCBF_RXPACK_FLOATS, pk);
+        geom_rx<GEO, FD>(pk, A.c0, pv, tau, apod, A.rc0);
     }
 }
 

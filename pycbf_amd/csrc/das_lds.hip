@@ -489,22 +489,11 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
         }
     };
     auto inner_eval = [&](int b, const float(&d)[NIN], float &tau, float &apod) {
-        typedef const __attribute__((address_space(4))) float cfloat;
         if (GEO == 0) {
             tau = d[0];
             apod = d[1];
-        } else if (OUTER_TX) {
-            cfloat *cp = (cfloat *)(A.rxpack + (int64_t)b * PYCBF_RXPACK_FLOATS);
-            float pk[PYCBF_RXPACK_FLOATS];
-#pragma unroll
-            for (int i = 0; i < PYCBF_RXPACK_FLOATS; ++i) pk[i] = cp[i];
-            geom_rx<GEO, true>(pk, A.c0, pv, tau, apod, A.rc0);
         } else {
-            cfloat *cp = (cfloat *)(A.txpack + (int64_t)b * PYCBF_TXPACK_FLOATS);
-            float pk[PYCBF_TXPACK_FLOATS];
-#pragma unroll
-            for (int i = 0; i < PYCBF_TXPACK_FLOATS; ++i) pk[i] = cp[i];
-            geom_tx<GEO, true>(pk, A.c0, pv, tau, apod, A.rc0);
+            elem_eval<GEO, true>(A, !OUTER_TX, b, p, pv, tau, apod);
         }
     };
 
