@@ -84,12 +84,11 @@ __device__ __forceinline__ void batch_item(const BatchReqDev &R, int tile, int r
         const int ix = tx0 + pass * CPP + lane / TILE, iz = tz0 + lane % TILE;
         if (tx0 + pass * CPP >= R.nx) break;  // uniform: the rest of the tile is outside
         const bool act = ix < R.nx && iz < R.nz;
-        const int64_t p = (int64_t)ix * R.nz + iz;
-        float pv[3] = {0.0f, 0.0f, 0.0f};
-        if (act) {
-            pv[0] = R.pvec[2 * p];
-            pv[1] = R.pvec[2 * p + 1];
-        }
+        const int64_t p = act ? (int64_t)ix * R.nz + iz : 0;
+        // every lane loads (pixel 0 past the grid), then a full wait: no load
+        // of the position is still counted when the receive loop's gathers are
+        float pv[3] = {R.pvec[2 * p], R.pvec[2 * p + 1], 0.0f};
+        __builtin_amdgcn_s_waitcnt(0);
         float acc = 0.0f;
 #pragma unroll 1
         for (int t0 = 0; t0 < R.ntx; t0 += BATCH_TB) {

@@ -81,11 +81,12 @@ __global__ __launch_bounds__(256) void k_das_reg(DasArgs A, int64_t ntiles, int 
     const int64_t i = tile * 64 + lane;
     const bool act = i < A.np;
     const int64_t p = act ? (A.perm ? (int64_t)A.perm[i] : i) : 0;
+    // every lane loads (p is 0 past the end), then a full wait: no load of the
+    // position is still counted when the receive loop's gathers are
     float pv[3] = {0.0f, 0.0f, 0.0f};
-    if (GEO > 0 && act) {
 #pragma unroll
-        for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
-    }
+    for (int d = 0; d < GEO; ++d) pv[d] = A.pvec[p * GEO + d];
+    __builtin_amdgcn_s_waitcnt(0);
     const int64_t rowlen = A.ch_stride;
     float acc = 0.0f;
 #pragma unroll 1
