@@ -201,28 +201,11 @@ __device__ __forceinline__ long long fixed_ll(double x)
 // adjoint: one lane per ray.  Rays are visited in a scrambled order
 // (q -> q * stride mod total, stride coprime to total) and each lane starts
 // its ray at a different step, so that the lanes of a wave follow unrelated
-// rays at different depths; along its ray a lane accumulates the bilinear
-// weights of the current cell in registers and adds them to the LDS map only
-// when the ray leaves the cell.  Each workgroup adds its LDS map into the
-// global 64-bit sums with integer atomics (exact, so the order does not
-// matter), and k_ray_reduce scales them.
-__device__ __forceinline__ void flush_cell(unsigned long long *gm, int nxm, int nzm, int i0, int k0,
-                                           float gl, double scale, float a00, float a01,
-                                           float a10, float a11)
-{
-    if (i0 < 0) return;
-    const int i1 = min(i0 + 1, nzm - 1), k1 = min(k0 + 1, nxm - 1);
-    // weights that stayed zero over the run (clamped edges) add nothing
-    auto add = [&](int idx, float w) {
-        if (w != 0.0f)
-            atomicAdd(&gm[idx], (unsigned long long)fixed_ll((double)(gl * w) * scale));
-    };
-    add(i0 * nxm + k0, a00);
-    add(i0 * nxm + k1, a01);
-    add(i1 * nxm + k0, a10);
-    add(i1 * nxm + k1, a11);
-}
-
+// rays at different depths; every step adds its four bilinear contributions
+// to the LDS map (branch-free across the wave: a run-length accumulation per
+// cell flushed when the ray left it diverged on almost every step).  Each
+// workgroup adds its LDS map into the global 64-bit sums with integer
+// atomics (exact, so the order does not matter), and k_ray_reduce scales them.
 __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigned *mbits,
                                                         unsigned long long *gsum, int64_t stride,
                                                         int64_t step)
@@ -251,11 +234,10 @@ __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigne
             float qx, L, fire;
             ray_of(e, a.ntx, a.steer, a.xorig, a.xele, a.c0, px, pz, qx, L, fire);
             const float gl = g * L * rn;
-            int ci = -1, ck = -1;
-            float a00 = 0.0f, a01 = 0.0f, a10 = 0.0f, a11 = 0.0f;
+            const double gls = (double)gl * scale;
+            const int nxm = a.g.nxm, nzm = a.g.nzm;
             // lane l walks its ray from step l (mod nsteps) on, so the lanes of
-            // a wave sit at different depths at any moment; the wrap costs one
-            // extra flush
+            // a wave sit at different depths (and nodes) at any moment
             const int mo = (int)(threadIdx.x & 63) % a.nsteps;
             for (int mm = 0; mm < a.nsteps; ++mm) {
                 const int m = mm + mo < a.nsteps ? mm + mo : mm + mo - a.nsteps;
@@ -264,19 +246,18 @@ __global__ __launch_bounds__(256) void k_ray_delays_bwd(RayArgs a, const unsigne
                 int i0, k0;
                 float wz, wx;
                 bilin(a.g, x, z, i0, k0, wz, wx);
-                if (i0 != ci || k0 != ck) {
-                    flush_cell(gm64, a.g.nxm, a.g.nzm, ci, ck, gl, scale, a00, a01, a10, a11);
-                    ci = i0;
-                    ck = k0;
-                    a00 = a01 = a10 = a11 = 0.0f;
-                }
+                const int i1 = min(i0 + 1, nzm - 1), k1 = min(k0 + 1, nxm - 1);
                 const float uz = 1.0f - wz, ux = 1.0f - wx;
-                a00 += uz * ux;
-                a01 += uz * wx;
-                a10 += wz * ux;
-                a11 += wz * wx;
+                // weights that are zero (clamped edges) add nothing
+                auto add = [&](int idx, float w) {
+                    if (w != 0.0f)
+                        atomicAdd(&gm64[idx], (unsigned long long)fixed_ll((double)w * gls));
+                };
+                add(i0 * nxm + k0, uz * ux);
+                add(i0 * nxm + k1, uz * wx);
+                add(i1 * nxm + k0, wz * ux);
+                add(i1 * nxm + k1, wz * wx);
             }
-            flush_cell(gm64, a.g.nxm, a.g.nzm, ci, ck, gl, scale, a00, a01, a10, a11);
         }
     }
     __syncthreads();
