@@ -57,6 +57,8 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
     const bool fine = A.ttx_lo || A.trx_lo;
     const float *la_t = OUTER_TX ? A.ttx_lo : A.trx_lo;
     const float *lb_t = OUTER_TX ? A.trx_lo : A.ttx_lo;
+    // positions in samples from double delays: products with 1 / dt (double)
+    const double rdt_d = 1.0 / (double)A.dt;
     for (int a = a0; a < a1; ++a) {
         const float ta = ta_t[(int64_t)a * np + p];
         const float wa = wa_t[(int64_t)a * np + p];
@@ -74,8 +76,7 @@ __global__ __launch_bounds__(256) void k_das(DasArgs A)
             const bool use = fabsf(w) >= A.thresh;
             if (use && fine) {
                 const double lb = lb_t ? (double)lb_t[(int64_t)b * np + p] : 0.0;
-                const double q = (((double)ta + (double)tb) + (la + lb) - (double)A.t0) /
-                                 (double)A.dt;
+                const double q = (((double)ta + (double)tb) + (la + lb) - (double)A.t0) * rdt_d;
                 gather_q<GK, FC>(A, ch0, fstride, nf, x, q, ok, v);
             } else if (use) {
                 gather<GK, FC>(A, ch0, fstride, nf, x, ok, v);

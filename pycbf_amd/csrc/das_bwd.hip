@@ -205,6 +205,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
     const bool fine = GEO == 0 && (A.ttx_lo || A.trx_lo) && GK != GK_NEAREST;
     const double lo_o = (fine && lo_o_t && act) ? (double)lo_o_t[(int64_t)o * A.np + p] : 0.0;
     float gt = 0.0f, ga = 0.0f;
+    const double rdt_d = 1.0 / (double)A.dt;  // positions in samples: products with 1 / dt
     for (int i = 0; act && i < ninner; ++i) {
         float tau_i = 0.0f, ap_i = 0.0f;
         if (GEO == 0) {
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(256) void k_das_bwd_tau(BwdArgs A)
         if (fine) {
             const double lo_i = lo_i_t ? (double)lo_i_t[(int64_t)i * A.np + p] : 0.0;
             const double xd = ((double)tau_o + (double)tau_i) + (lo_o + lo_i);
-            q = (xd - (double)A.t0) / (double)A.dt;
+            q = (xd - (double)A.t0) * rdt_d;
         }
         for (int f = 0; f < A.nfr; ++f) {
             const int64_t row = ((int64_t)f * A.ntx + tx) * A.nrx + rx;
