@@ -401,8 +401,12 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
 // slots, so chunk q of the staged block lands at byte 16 q.  Per outer
 // element one pass computes every inner channel's window into an LDS table;
 // channel b + 1 is in flight while b is evaluated (two LDS buffers, one
-// barrier per staged channel).  A window wider than the slot takes direct
-// global gathers, as in k_das_lds.  The quotients by c0 (geometry) and dt
+// barrier per staged channel).  Nothing in the evaluation of channel b waits
+// on the DMA of channel b + 1: the buffers are separate LDS objects (a read
+// of one is known not to alias the DMA into the other), geometry packs come
+// through the scalar cache and table entries are fetched a channel ahead.
+// Windows wider than the slot take direct global gathers, in a loop after
+// the staged channels, as in k_das_lds.  The quotients by c0 (geometry) and dt
 // (knot index) are products with the rounded reciprocals plus one fused
 // correction (div_rn), which the launch takes only for divisors whose every
 // quotient it rounds like the division (recip_division_exact): the same
