@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PYCBF_ABI_VERSION 2
+#define PYCBF_ABI_VERSION 3
 #define PYCBF_EINVAL 1001
 
 /* interp kinds: 0..3 keep pycbf's interp_keys (pycbf/gpu/__core_bf_classes__.py:215,332) */
@@ -70,7 +70,7 @@ extern "C" {
 
 /* floats per transmit / receive element in the synthetic geometry packs */
 #define PYCBF_TXPACK_FLOATS 16
-#define PYCBF_RXPACK_FLOATS 8
+#define PYCBF_RXPACK_FLOATS 12
 
 /* struct xInfo, pycbf/gpu/__engines__.cu:360-364 (12 bytes) */
 typedef struct pycbf_xinfo {
@@ -132,7 +132,9 @@ int pycbf_das_tabbed_residual(const pycbf_rfinfo *info, const float *rf, int64_t
  * from pycbf's Synthetic parameters (pycbf/__bf_base_classes__.py:141-183, all
  * host arrays).  The acceptance tests "acos(v) > ala" of
  * pycbf/gpu/__engines__.cu:412,427,468 become exact comparisons v < T with T
- * derived from ala in double precision.
+ * derived from ala in double precision; each receive pack also carries, as a
+ * double in floats 8-9, the midpoint between T and the float below it, with
+ * which the receive test runs as proj < m * mag without a division.
  *   txpack: ntx * PYCBF_TXPACK_FLOATS floats, rxpack: nrx * PYCBF_RXPACK_FLOATS. */
 int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float *nvectx,
                          const float *t0tx, const float *alatx, const float *doftx, int64_t nrx,

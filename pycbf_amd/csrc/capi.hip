@@ -321,6 +321,17 @@ static float acos_threshold(float ala)
     return T;
 }
 
+// The double midpoint between T and the float below it, the bound with which
+// the receive kernels decide v < T on proj / mag without dividing
+// (RN(proj / mag) < T <=> proj < m * mag); T = +inf stores 1 + 2^-24 (the
+// bound of RN(q) <= 1), T = -inf stores -inf.
+static double acos_midpoint(float T)
+{
+    if (T == INFINITY) return 1.0 + 0x1p-24;
+    if (T == -INFINITY) return -INFINITY;
+    return 0.5 * ((double)nextafterf(T, -INFINITY) + (double)T);
+}
+
 // A plan whose partial images do not fit `cap` bytes takes fewer, larger
 // outer groups (down to one, no partials); the images are the same sums in
 // another order.
@@ -518,6 +529,8 @@ int pycbf_synthetic_pack(int ndim, int64_t ntx, const float *ovectx, const float
             pk[3 + d] = nvecrx[e * ndim + d];
         }
         pk[6] = acos_threshold(alarx[e]);
+        const double m = acos_midpoint(pk[6]);
+        memcpy(pk + 8, &m, sizeof m);
     }
     return 0;
 }

@@ -146,6 +146,46 @@ __device__ __forceinline__ void geom_tx(const float *__restrict__ pk, float c0, 
     apod = (((double)mag > 1e-12) && acos_gt(fabsf(proj / mag), T)) ? 0.0f : 1.0f;
 }
 
+// sqrtf(x), correctly rounded.  For x >= 2^-96 (+inf included) it is the
+// hardware root (within one ulp) moved to the neighbour the residuals
+// x - s' * s select: the instructions sqrtf itself runs for such x, without its
+// rescaling of tiny inputs and its zero / infinity select.  Smaller x and NaN
+// take sqrtf in a branch the compiler cannot flatten (the empty volatile asm
+// keeps it from computing both sides and selecting).
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = (__builtin_fmaf(-sd, s, x) <= 0.0f) ? sd : s;
+    r = (__builtin_fmaf(-su, s, x) > 0.0f) ? su : r;
+    if (!(x >= 0x1p-96f)) {
+        asm volatile("");
+        r = sqrtf(x);
+    }
+    return r;
+}
+
+// The receive weight rule acos(RN(proj / mag)) > ala, decided without the
+// division.  m (rx pack floats 8-9, a double) is the midpoint between the
+// pack's T and the float below it; for mag > 0:
+//   RN(proj / mag) < T   <=>  proj < m * mag
+//   RN(proj / mag) >= -1 <=>  proj > -(1 + 2^-24) * mag
+// (T <= 1, so RN < T also gives RN <= 1).  Both products are exact in double
+// (25 x 24 significant bits) and no float proj equals either one: the midpoints
+// have 25 significant bits (an odd numerator; T is never subnormal, as
+// |cos(a)| >= 4e-8 for every float angle a in [0, pi]) and a product with a
+// float keeps them, where proj has 24.  T = +inf (ala < 0: only |q| <= 1 is
+// asked) stores m = 1 + 2^-24, T = -inf stores m = -inf.  The same holds at
+// mag = +inf (proj finite: RN(q) = +-0; proj infinite: NaN, no weight) and for
+// NaN operands (no weight).
+__device__ __forceinline__ bool rx_outside(float proj, float mag, const float *pk)
+{
+    const double m = __hiloint2double(__float_as_int(pk[9]), __float_as_int(pk[8]));
+    const double md = (double)mag, pd = (double)proj;
+    return (md > 1e-12) && (pd > md * -0x1.000001p0) && (pd < md * m);
+}
+
 template <int NDIM, bool FD = false>
 __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, const float *p,
                                         float &tau, float &apod, float rc0 = 0.0f)
@@ -155,11 +195,11 @@ __device__ __forceinline__ void geom_rx(const float *__restrict__ pk, float c0, 
     for (int d = 0; d < NDIM; ++d) {
         const float di = p[d] - pk[d];
         mag += di * di;
-        proj += di * pk[3 + d];
+        proj = (d == 0) ? di * pk[3] : proj + di * pk[3 + d];
     }
-    mag = sqrtf(mag);
+    mag = sqrt_rn(mag);
     tau = FD ? div_rn(mag, c0, rc0) : mag / c0;
-    apod = (((double)mag > 1e-12) && acos_gt(proj / mag, pk[6])) ? 0.0f : 1.0f;
+    apod = rx_outside(proj, mag, pk) ? 0.0f : 1.0f;
 }
 
 // ---------------------------------------------------------------------------

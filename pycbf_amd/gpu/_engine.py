@@ -40,8 +40,8 @@ SLSC_PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 # float above 0.01 (PYCBF_GPU_REFERENCE_THRESH)
 GPU_REFERENCE_THRESH = 0.010000000707805157
 TXPACK_FLOATS = 16
-RXPACK_FLOATS = 8
-ABI_VERSION = 2
+RXPACK_FLOATS = 12
+ABI_VERSION = 3
 
 
 class xInfo(ct.Structure):

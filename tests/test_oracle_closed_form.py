@@ -196,6 +196,42 @@ def test_acos_threshold_predicate_is_exact():
         assert np.array_equal(dev, ref), (ala, T)
 
 
+def test_receive_midpoint_rule_matches_float_division():
+    """The receive kernels decide RN(proj / mag) in [-1, 1] and < T without the
+    division: proj > -(1 + 2^-24) mag and proj < m mag in double, with m the
+    midpoint stored at rx pack floats 8-9.  Against numpy's IEEE float32
+    quotient for proj one or a few ulps either side of T * mag and -mag, for
+    mag over many binades, the special thresholds and non-finite operands."""
+    rng = np.random.default_rng(16)
+    alas = np.concatenate([rng.uniform(0, math.pi, 60), [0.0, 1e-6, math.pi / 2,
+                                                         np.float32(math.pi), 3.2, -0.1,
+                                                         np.nan]]).astype(np.float32)
+    _, rxp = _pack(alarx=alas)
+    m_all = np.ascontiguousarray(rxp[:, 8:10]).view(np.float64)[:, 0]
+    lo = -(1.0 + 2.0 ** -24)
+    mag = (rng.uniform(1, 2, 400) * 2.0 ** rng.integers(-38, 60, 400)).astype(np.float32)
+    mag = np.concatenate([mag, np.float32([1e-12, 1.0000001e-12, 1, 2, np.inf, np.nan])])
+    for T, m in zip(rxp[:, 6], m_all):
+        anchors = [np.float32(T) if np.isfinite(T) else np.float32(1), np.float32(-1)]
+        props = []
+        for a in anchors:
+            base = (np.float64(a) * mag.astype(np.float64)).astype(np.float32)
+            for k in range(-3, 4):
+                step = base.copy()
+                for _ in range(abs(k)):
+                    step = np.nextafter(step, np.float32(np.inf if k > 0 else -np.inf))
+                props.append(np.stack([step, mag], 1))
+        pm = np.concatenate(props + [np.float32([[np.inf, np.inf], [-np.inf, np.inf],
+                                                  [np.nan, 1], [3, np.inf], [-3, np.inf]])])
+        proj, mg = pm[:, 0], pm[:, 1]
+        with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+            q = proj / mg
+            ref = (mg.astype(np.float64) > 1e-12) & (q >= -1) & (q <= 1) & (q < T)
+            pd, md = proj.astype(np.float64), mg.astype(np.float64)
+            dev = (md > 1e-12) & (pd > md * lo) & (pd < md * m)
+        assert np.array_equal(dev, ref), (T, np.flatnonzero(dev != ref)[:5])
+
+
 def test_geometry_closed_forms():
     """Plane wave tau = <p-o, n>/c, receive tau = |p-e|/c, receive apod from
     the acceptance cone, focused tau = sign(proj)|p-o|/c + t0."""
