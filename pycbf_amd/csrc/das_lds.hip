@@ -415,6 +415,11 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_lds(DasArgs A)
 constexpr int DMA_MAXB = 256;    // inner channels per outer element (window table)
 constexpr int DMA_CHUNKS = 768;  // 16-byte chunks per buffer (12 KB), all frames
 
+// one (y, h*y2) table entry as read from a staging buffer
+struct __attribute__((aligned(8))) YZ {
+    float y, z;
+};
+
 typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
@@ -456,6 +461,14 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
     const float *bnd = A.bounds + (int64_t)bx * (A.ntx + A.nrx) * 2;
     const int boff_a = OUTER_TX ? 0 : A.ntx;
     const int boff_b = OUTER_TX ? A.ntx : 0;
+    // A ds_read2_b64 reaches 255 8-byte slots past its base: FG frame rows.
+    // The frames' reads take one base per group of FG rows, stepped by
+    // fg_step, a value the compiler cannot see (the empty asm), so it adds
+    // it once per group; with a known constant it recomputes the address of
+    // every row that its offsets cannot reach.
+    constexpr int FG = (2032 / (CH * 16) + 1 < FC) ? 2032 / (CH * 16) + 1 : FC;
+    int fg_step = FG * CH * 2;  // in (y, h*y2) entries
+    asm("" : "+v"(fg_step));
 
     // pixel position (every lane: p is 0 past the end), and a full wait, so no
     // load of the prologue is still counted when the channel loop's DMAs are
@@ -582,15 +595,21 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             // every frame slot: those of frames past nf hold copies of frame
             // nf - 1 and feed sums that are never stored; straight-line code
             // lets the LDS reads of all frames overlap
-            const float *L = (buf ? lds1 : lds0) + (j - w.x) * 2;
+            const YZ *L = (const YZ *)(buf ? lds1 : lds0) + (j - w.x);
 #pragma unroll
-            for (int f = 0; f < FC; ++f) {
-                const int par = FODD ? par0 ^ (f & 1) : par0;
-                const float *Ef = L + f * CH * 4 + par * 2;
-                float r = __builtin_fmaf(ka, Ef[0], acc[f]);
-                r = __builtin_fmaf(kb, Ef[2], r);
-                r = __builtin_fmaf(kc, Ef[1], r);
-                acc[f] = __builtin_fmaf(kd, Ef[3], r);
+            for (int g = 0; g < FC; g += FG) {
+                const YZ *Lg = L + (g / FG) * fg_step;
+#pragma unroll
+                for (int k = 0; k < FG && g + k < FC; ++k) {
+                    const int f = g + k;
+                    const int par = FODD ? par0 ^ (f & 1) : par0;
+                    const YZ *Ef = Lg + k * CH * 2 + par;
+                    const YZ e0 = Ef[0], e1 = Ef[1];
+                    float r = __builtin_fmaf(ka, e0.y, acc[f]);
+                    r = __builtin_fmaf(kb, e1.y, r);
+                    r = __builtin_fmaf(kc, e0.z, r);
+                    acc[f] = __builtin_fmaf(kd, e1.z, r);
+                }
             }
         };
 
