@@ -545,16 +545,20 @@ __global__ __launch_bounds__(LDS_BLOCK) void k_das_cubic_dma(DasArgs A)
             constexpr int buf = decltype(bufc)::value;
             if (!staged(b)) return;
             const int2 w = wins[b];
-            // entry w.x of the channel's row, offset from the frame's table and
+            // the window is the same in every lane (one LDS word): scalar
+            // registers, so the channel's offset is scalar arithmetic
+            const int wx = __builtin_amdgcn_readfirstlane(w.x);
+            const int wy = __builtin_amdgcn_readfirstlane(w.y);
+            // entry wx of the channel's row, offset from the frame's table and
             // its parity in frame 0 (workgroup-uniform)
-            const int64_t cho = row_of(b, -f0) * rowlen + w.x;
+            const int64_t cho = row_of(b, -f0) * rowlen + wx;
             const int cpar = (int)(cho & 1);
 #pragma unroll
             for (int k = 0; k < NI; ++k) {
-                // the row's first staged entry is w.x - par, 16-byte aligned
-                // (row r-1's last if -1); its nch = (w.y + par + 1) / 2 chunks
+                // the row's first staged entry is wx - par, 16-byte aligned
+                // (row r-1's last if -1); its nch = (wy + par + 1) / 2 chunks
                 const int par = FODD ? (dma_par[k] ^ cpar) : cpar;
-                const int c = min(dma_c[k], (w.y + par - 1) >> 1);
+                const uint32_t c = (uint32_t)min(dma_c[k], (wy + par - 1) >> 1);
                 const float *src = dma_src[k] + (cho - par) * 2 + 4 * c;
                 float *dst = (buf ? lds1 : lds0) + (k * (LDS_BLOCK / 64) + wave) * 64 * 4;
                 __builtin_amdgcn_global_load_lds((glb_void *)src, (lds_void *)dst, 16, 0, 0);
