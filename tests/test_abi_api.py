@@ -60,6 +60,19 @@ def test_header_constants_match_binding():
     assert E.RFInfo.tInfo.offset == 20 and E.RFInfo.ndim.offset == 16
 
 
+def test_integration_stub_matches_header():
+    """The reference-side binding in INTEGRATION.md sizes the geometry packs
+    and checks the ABI version with constants of its own."""
+    text = open(HEADER).read()
+    consts = dict((k, int(v)) for k, v in re.findall(r"#define (PYCBF_\w+) (\d+)", text))
+    stub = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"_TXPACK, _RXPACK, _ABI = (\d+), (\d+), (\d+)", stub)
+    assert m, "pack constants not found in the stub"
+    assert int(m.group(1)) == consts["PYCBF_TXPACK_FLOATS"]
+    assert int(m.group(2)) == consts["PYCBF_RXPACK_FLOATS"]
+    assert int(m.group(3)) == consts["PYCBF_ABI_VERSION"]
+
+
 def test_rfinfo_layout_matches_reference_dtype():
     """struct RFInfo as the reference's numpy dtype lays it out
     (pycbf/gpu/__engines__.py:13-14)."""
