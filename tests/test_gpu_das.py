@@ -223,6 +223,47 @@ def test_synthetic_tables_bitwise(mode):
     assert ref[3].min() == 0.0
 
 
+def test_receive_boundary_and_tiny_distances_bitwise():
+    """The receive weight rule and delay at their edge cases, bit for bit
+    against the oracle: for four receive elements (acceptance angles 0.32,
+    0.5, 0.1 and 1.2 rad, tilted normals), points on arcs through both cone
+    edges in steps of 3e-8 rad at 5, 17 and 33 mm, so that the float32
+    cosine proj / mag lands on the element's threshold T and on every float
+    a few ulps either side of it; and points at the element itself and
+    1e-17 .. 4e-12 m away, whose squared distance lies below 2^-96 (the
+    square root's rescaled branch) or just above it."""
+    alas = np.float32([0.32175055, 0.5, 0.1, 1.2])
+    phis = np.array([0.0, 0.2, -0.15, 0.05])
+    xo = np.float32([-3e-3, -1e-3, 1e-3, 3e-3])
+    ovecrx = np.stack([xo, np.zeros(4, np.float32)], 1).astype(np.float32)
+    nvecrx = np.stack([np.sin(phis), np.cos(phis)], 1).astype(np.float32)
+    pts = []
+    for e in range(4):
+        for r in (5e-3, 17e-3, 33e-3):
+            for sgn in (1.0, -1.0):
+                th = phis[e] + sgn * (float(alas[e]) + np.arange(-150, 151) * 3e-8)
+                pts.append(np.stack([xo[e] + r * np.sin(th), r * np.cos(th)], 1))
+        o = ovecrx[e].astype(np.float64)
+        for off in (0.0, 1e-17, 3e-16, 2e-15, 1e-14, 4e-12):
+            pts.append(np.array([[o[0] + off, o[1] + off], [o[0] - off, o[1] + 0.5 * off]]))
+    p = dict(ovectx=np.zeros((1, 2), np.float32), nvectx=np.float32([[0.0, 1.0]]),
+             t0tx=np.zeros(1, np.float32), alatx=np.zeros(1, np.float32),
+             doftx=np.zeros(1, np.float32), ovecrx=ovecrx, nvecrx=nvecrx, alarx=alas, c0=1540.0,
+             pnts=np.concatenate(pts).astype(np.float32))
+    ref = oracle.synthetic_tables(p["ovectx"], p["nvectx"], p["t0tx"], p["alatx"], p["doftx"],
+                                  p["ovecrx"], p["nvecrx"], p["alarx"], p["c0"], p["pnts"])
+    # the cases are there: both weights for every element, and squared
+    # distances below 2^-96
+    assert all(0.0 in ref[3][e] and 1.0 in ref[3][e] for e in range(4))
+    d = p["pnts"][None] - p["ovecrx"][:, None]
+    assert int(((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) < 2.0 ** -96).sum()) >= 16
+    bf = SyntheticBeamformer(**p, t0=0.0, dt=1.0 / sim.FS, nt=1024, interp={"kind": "cubic"})
+    got = [t.cpu().numpy() for t in bf.tables()]
+    for n, g, r in zip(["tautx", "apodtx", "taurx", "apodrx"], got, ref):
+        bad = g.view(np.uint32) != r.view(np.uint32)
+        assert not bad.any(), f"{n}: {int(bad.sum())} differ"
+
+
 @pytest.mark.parametrize("mode", SYN_MODES)
 @pytest.mark.parametrize("interp,kind,usf", KINDS, ids=[k[1] + str(k[2]) for k in KINDS])
 def test_synthetic_das_matches_oracle(mode, interp, kind, usf):
